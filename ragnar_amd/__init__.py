@@ -1,0 +1,23 @@
+"""MI355X-native (gfx950) radiation hot path of ragnar.
+
+``ragnar_amd.ragnar`` is a drop-in for the reference's ``ragnar`` module
+(``from ragnar_amd import ragnar as rg``); ``ragnar_amd.torch_ops`` exposes the
+same kernels on torch device tensors, including the differentiable
+``SynchrotronSpectrumFn``.  Both sit on libragnar_hip.so, whose C ABI is
+declared in include/ragnar_hip.h.  There is no CPU execution path: without the
+compiled library the import fails.
+"""
+from __future__ import annotations
+
+from pathlib import Path
+
+_HERE = Path(__file__).resolve().parent
+
+try:
+    from . import ragnar  # noqa: F401  (pybind11 module, links libragnar_hip.so)
+except ImportError as exc:  # pragma: no cover - exercised only when unbuilt
+    raise ImportError(
+        "ragnar_amd: the compiled module is missing or stale; build it with "
+        "`python ragnar_amd/build.py` (or __graft_entry__.build())") from exc
+
+__all__ = ["ragnar"]

@@ -1,0 +1,627 @@
+// Synchrotron emission on gfx950: per-particle spectrum (forward), its adjoint,
+// and the spectrum of a tabulated distribution.
+//
+// Work layout (forward).  A block of 256 threads (4 waves) walks tiles of 256
+// sources.  Each thread turns one source into (W, V_hi, V_lo) in FP64 and parks
+// it in LDS; then each wave takes 64 of the tile's sources and every lane
+// evaluates them against its own photon bins (bin e = lane + 64k, k < NB), so
+// the per-source operands are wave-uniform broadcasts and each bin accumulates
+// in a register: no atomics.  A lane's 64-term float partial is folded into a
+// double per tile; the four waves and then the blocks are combined in a fixed
+// order, so results are bitwise reproducible.  The F table lives in LDS as
+// (c_j, s_j) segments; one eval is 6 VALU ops + one ds_read_b64.
+#include <algorithm>
+#include <cmath>
+#include <limits>
+#include <vector>
+
+#include "emission.hpp"
+#include "rgn_core.hpp"
+
+namespace rgn {
+
+const std::vector<float>& ftable_y(const rgn_ftable* t);
+float ftable_xmin(const rgn_ftable* t);
+float ftable_xmax(const rgn_ftable* t);
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kMaxTableEntries = 12288;  // 96 KiB of (c, s) pairs in LDS
+
+// ---------------------------------------------------------------------------
+// Exact-product representation of the per-bin coordinate U_e: search c near 1
+// such that S = float(U / c) makes c * S closest to U.  One wave per bin.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(64) product_pair_kernel(const double* __restrict__ U,
+                                                          float2* __restrict__ out, int n) {
+  const int e = blockIdx.x;
+  if (e >= n) return;
+  const double target = U[e];
+  double best_r = INFINITY;
+  int best_k = 0;
+  float best_c = 1.0f, best_s = static_cast<float>(target);
+  // k in [-2047, 2047]: c = 1 + k 2^-23 above 1, 1 + k 2^-24 below 1
+  for (int k = static_cast<int>(threadIdx.x) - 2047; k <= 2047; k += 64) {
+    const float c = (k >= 0) ? 1.0f + static_cast<float>(k) * 0x1p-23f
+                             : 1.0f + static_cast<float>(k) * 0x1p-24f;
+    const float s = static_cast<float>(target / static_cast<double>(c));
+    const double r = fabs(target - static_cast<double>(c) * static_cast<double>(s));
+    if (r < best_r || (r == best_r && abs(k) < abs(best_k))) {
+      best_r = r;
+      best_k = k;
+      best_c = c;
+      best_s = s;
+    }
+  }
+  __shared__ double rs[64];
+  __shared__ int ks[64];
+  rs[threadIdx.x] = best_r;
+  ks[threadIdx.x] = best_k;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int arg = 0;
+    for (int i = 1; i < 64; ++i) {
+      if (rs[i] < rs[arg] || (rs[i] == rs[arg] && abs(ks[i]) < abs(ks[arg]))) arg = i;
+    }
+    ks[0] = arg;
+  }
+  __syncthreads();
+  if (static_cast<int>(threadIdx.x) == ks[0]) out[e] = make_float2(best_c, best_s);
+}
+
+// ---------------------------------------------------------------------------
+// Shared inner loop: 64 sources of `src` against the lane's NB bins.
+// ---------------------------------------------------------------------------
+template <int NB>
+__device__ __forceinline__ void eval_sources(const float4* __restrict__ src, int count,
+                                             const float (&c)[NB], const float (&S)[NB],
+                                             const float2* __restrict__ ltab, float (&acc)[NB]) {
+#pragma unroll 4
+  for (int j = 0; j < count; ++j) {
+    const float4 q = src[j];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+      const float u = fmaf(c[k], S[k], q.y) + q.z;
+      const float2 cs = ltab[cvt_sat_u32(u)];
+      acc[k] = fmaf(q.x, fmaf(u, cs.y, cs.x), acc[k]);
+    }
+  }
+}
+
+struct ParticleSourceFn {
+  ParticlesDev D;
+  __device__ Source operator()(size_t p, const PlanDev& P) const {
+    return particle_source(D, p, P);
+  }
+};
+
+// synchrotron.hpp:72-95: eps = e_at * gb * gb / (g_syn * g_syn) in float;
+// the weight is f (times gb for log-spaced distribution bins).
+struct DistSourceFn {
+  const float* gbeta;
+  const float* f;
+  int dist_log;
+  __device__ Source operator()(size_t g, const PlanDev& P) const {
+    const float gb = gbeta[g];
+    const float eps = P.e_at * gb * gb / (P.g_syn * P.g_syn);
+    const float W = dist_log ? f[g] * gb : f[g];
+    return make_source(W, eps, P);
+  }
+};
+
+template <int NB, class SourceFn>
+__global__ void __launch_bounds__(kThreads) forward_kernel(SourceFn fn, size_t n, PlanDev P,
+                                                           double* __restrict__ partials) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  float4* tile = reinterpret_cast<float4*>(smem);
+  float2* ltab = reinterpret_cast<float2*>(smem + kThreads * sizeof(float4));
+  for (int j = threadIdx.x; j < P.TP; j += kThreads) ltab[j] = P.table[j];
+
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  float c[NB], S[NB];
+  double dacc[NB];
+#pragma unroll
+  for (int k = 0; k < NB; ++k) {
+    const float2 cs = P.bincs[lane + 64 * k];
+    c[k] = cs.x;
+    S[k] = cs.y;
+    dacc[k] = 0.0;
+  }
+
+  const size_t stride = static_cast<size_t>(gridDim.x) * kThreads;
+  for (size_t base = static_cast<size_t>(blockIdx.x) * kThreads; base < n; base += stride) {
+    const size_t p = base + threadIdx.x;
+    Source s{0.0f, P.v_safe, 0.0f};
+    if (p < n) s = fn(p, P);
+    __syncthreads();  // the previous tile has been consumed
+    tile[threadIdx.x] = make_float4(s.W, s.vhi, s.vlo, 0.0f);
+    __syncthreads();
+    float acc[NB];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) acc[k] = 0.0f;
+    eval_sources<NB>(tile + wave * 64, 64, c, S, ltab, acc);
+#pragma unroll
+    for (int k = 0; k < NB; ++k) dacc[k] += static_cast<double>(acc[k]);
+  }
+
+  // Fixed-order combine of the four waves, then one column of `partials`.
+  __syncthreads();
+  double* red = reinterpret_cast<double*>(smem);
+#pragma unroll
+  for (int k = 0; k < NB; ++k) red[(wave * NB + k) * 64 + lane] = dacc[k];
+  __syncthreads();
+  for (int e = threadIdx.x; e < 64 * NB; e += kThreads) {
+    const int k = e >> 6, l = e & 63;
+    const double v = ((red[(0 * NB + k) * 64 + l] + red[(1 * NB + k) * 64 + l]) +
+                      red[(2 * NB + k) * 64 + l]) +
+                     red[(3 * NB + k) * 64 + l];
+    partials[static_cast<size_t>(e) * gridDim.x + blockIdx.x] = v;
+  }
+}
+
+// out[e] = e_e * sum_b partials[e][b] (fixed order), 0 for dead bins.
+__global__ void __launch_bounds__(kThreads) combine_kernel(const double* __restrict__ partials,
+                                                           int nblocks, const float* __restrict__ bine,
+                                                           int nbins, float* __restrict__ out) {
+  const int e = blockIdx.x;
+  if (e >= nbins) return;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += kThreads) s += partials[static_cast<size_t>(e) * nblocks + b];
+  __shared__ double red[kThreads];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (static_cast<int>(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const float ee = bine[e];
+    out[e] = (ee == 0.0f) ? 0.0f : static_cast<float>(static_cast<double>(ee) * red[0]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint.  Same tile walk; for each source the lane sums g_e F and g_e s over
+// its bins, the wave reduces the two sums with DPP, and the owning thread
+// turns (A, C) into gradients in FP64 (recomputing gamma and the pitch terms).
+// ---------------------------------------------------------------------------
+struct GradOut {
+  float* gw;
+  float* gu[3];
+  float* gb[3];
+  float* ge[3];
+};
+
+template <int NB>
+__global__ void __launch_bounds__(kThreads) adjoint_kernel(ParticlesDev D, size_t n, PlanDev P,
+                                                           const float* __restrict__ gbar,
+                                                           GradOut G) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  float4* tile = reinterpret_cast<float4*>(smem);
+  float2* ac = reinterpret_cast<float2*>(smem + kThreads * sizeof(float4));
+  float2* ltab = reinterpret_cast<float2*>(smem + kThreads * (sizeof(float4) + sizeof(float2)));
+  for (int j = threadIdx.x; j < P.TP; j += kThreads) ltab[j] = P.table[j];
+
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  float c[NB], S[NB], g[NB];
+#pragma unroll
+  for (int k = 0; k < NB; ++k) {
+    const int e = lane + 64 * k;
+    const float2 cs = P.bincs[e];
+    c[k] = cs.x;
+    S[k] = cs.y;
+    g[k] = (e < P.nbins) ? gbar[e] * P.bine[e] : 0.0f;
+  }
+  const double dlnx = -P.K / 0.69314718055994530942;  // dF/dln x = s * du/dln x
+
+  const size_t stride = static_cast<size_t>(gridDim.x) * kThreads;
+  for (size_t base = static_cast<size_t>(blockIdx.x) * kThreads; base < n; base += stride) {
+    const size_t p = base + threadIdx.x;
+    Source s{0.0f, P.v_safe, 0.0f};
+    if (p < n) s = particle_source(D, p, P);
+    __syncthreads();
+    tile[threadIdx.x] = make_float4(s.W, s.vhi, s.vlo, 0.0f);
+    __syncthreads();
+    const float4* src = tile + wave * 64;
+    for (int j = 0; j < 64; ++j) {
+      const float4 q = src[j];
+      float a = 0.0f, cc = 0.0f;
+#pragma unroll
+      for (int k = 0; k < NB; ++k) {
+        const float u = fmaf(c[k], S[k], q.y) + q.z;
+        const float2 cs = ltab[cvt_sat_u32(u)];
+        a = fmaf(g[k], fmaf(u, cs.y, cs.x), a);
+        cc = fmaf(g[k], cs.y, cc);
+      }
+      a = wave_sum_to_lane63(a);
+      cc = wave_sum_to_lane63(cc);
+      if (lane == 63) ac[wave * 64 + j] = make_float2(a, cc);
+    }
+    __syncthreads();
+    if (p >= n) continue;
+    const float2 mine = ac[threadIdx.x];
+    const bool live = s.W != 0.0f;
+    const double ux = D.u[0][p], uy = D.u[1][p], uz = D.u[2][p];
+    const double ex = load_or_zero(D.e[0], p), ey = load_or_zero(D.e[1], p),
+                 ez = load_or_zero(D.e[2], p);
+    const double bx = D.b[0][p], by = D.b[1][p], bz = D.b[2][p];
+    const double w = D.w ? D.w[p] : 1.0;
+    double gw = 0, gux = 0, guy = 0, guz = 0, gbx = 0, gby = 0, gbz = 0, gex = 0, gey = 0,
+           gez = 0;
+    if (live) {
+      const double A = mine.x;
+      const double C = dlnx * mine.y;
+      const double gamma = sqrt(1.0 + ux * ux + uy * uy + uz * uz);
+      const double btx = ux / gamma, bty = uy / gamma, btz = uz / gamma;
+      const double bde = btx * ex + bty * ey + btz * ez;
+      const double vx = ex + (bty * bz - btz * by);
+      const double vy = ey + (btz * bx - btx * bz);
+      const double vz = ez + (btx * by - bty * bx);
+      const double rootQ = sqrt(vx * vx + vy * vy + vz * vz - bde * bde);
+      const double chi = rootQ / P.B0;
+      gw = chi * A;
+      const double gchi = w * (A - C);
+      const double ggamma = -2.0 * w * chi * C / gamma;
+      const double coef = gchi / (static_cast<double>(P.B0) * rootQ);
+      // dchi/dB = (V x beta) / (B0 sqrt Q)
+      gbx = coef * (vy * btz - vz * bty);
+      gby = coef * (vz * btx - vx * btz);
+      gbz = coef * (vx * bty - vy * btx);
+      // dchi/dE = (V - (beta.E) beta) / (B0 sqrt Q)
+      gex = coef * (vx - bde * btx);
+      gey = coef * (vy - bde * bty);
+      gez = coef * (vz - bde * btz);
+      // dchi/dbeta = (B x V - (beta.E) E) / (B0 sqrt Q)
+      const double gbtx = coef * ((by * vz - bz * vy) - bde * ex);
+      const double gbty = coef * ((bz * vx - bx * vz) - bde * ey);
+      const double gbtz = coef * ((bx * vy - by * vx) - bde * ez);
+      const double udg = ux * gbtx + uy * gbty + uz * gbtz;
+      const double g3 = gamma * gamma * gamma;
+      gux = gbtx / gamma - ux * udg / g3 + ggamma * ux / gamma;
+      guy = gbty / gamma - uy * udg / g3 + ggamma * uy / gamma;
+      guz = gbtz / gamma - uz * udg / g3 + ggamma * uz / gamma;
+    }
+    if (G.gw) G.gw[p] = static_cast<float>(gw);
+    if (G.gu[0]) G.gu[0][p] = static_cast<float>(gux);
+    if (G.gu[1]) G.gu[1][p] = static_cast<float>(guy);
+    if (G.gu[2]) G.gu[2][p] = static_cast<float>(guz);
+    if (G.gb[0]) G.gb[0][p] = static_cast<float>(gbx);
+    if (G.gb[1]) G.gb[1][p] = static_cast<float>(gby);
+    if (G.gb[2]) G.gb[2][p] = static_cast<float>(gbz);
+    if (G.ge[0]) G.ge[0][p] = static_cast<float>(gex);
+    if (G.ge[1]) G.ge[1][p] = static_cast<float>(gey);
+    if (G.ge[2]) G.ge[2][p] = static_cast<float>(gez);
+  }
+}
+
+// Per-particle (e_peak, chiR, W, V_hi, V_lo) for diagnostics.
+__global__ void __launch_bounds__(kThreads) particle_terms_kernel(ParticlesDev D, size_t n,
+                                                                  PlanDev P, float* __restrict__ out) {
+  const size_t p = blockIdx.x * static_cast<size_t>(kThreads) + threadIdx.x;
+  if (p >= n) return;
+  float e_peak, chiR;
+  omega_sync(D.u[0][p], D.u[1][p], D.u[2][p], load_or_zero(D.e[0], p), load_or_zero(D.e[1], p),
+             load_or_zero(D.e[2], p), D.b[0][p], D.b[1][p], D.b[2][p], P.B0, P.g_syn, P.e_at,
+             e_peak, chiR);
+  const Source s = particle_source(D, p, P);
+  out[5 * p + 0] = e_peak;
+  out[5 * p + 1] = chiR;
+  out[5 * p + 2] = s.W;
+  out[5 * p + 3] = s.vhi;
+  out[5 * p + 4] = s.vlo;
+}
+
+ParticlesDev make_particles(const float* const d_u[3], const float* const d_e[3],
+                            const float* const d_b[3], const float* d_w) {
+  ParticlesDev D{};
+  for (int k = 0; k < 3; ++k) {
+    RGN_REQUIRE(d_u && d_u[k], RGN_ERR_INVALID, "null momentum component");
+    RGN_REQUIRE(d_b && d_b[k], RGN_ERR_INVALID, "null magnetic-field component");
+    D.u[k] = d_u[k];
+    D.b[k] = d_b[k];
+    D.e[k] = d_e ? d_e[k] : nullptr;
+  }
+  D.w = d_w;
+  return D;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Plan
+// ---------------------------------------------------------------------------
+void build_sync_plan(SyncPlan& plan, const rgn_ftable* table, const float* h_bins, int nbins,
+                     float B0, float g_syn, float e_at) {
+  RGN_REQUIRE(table != nullptr, RGN_ERR_INVALID, "null F table");
+  RGN_REQUIRE(nbins > 0, RGN_ERR_INVALID, "no photon bins");
+  RGN_REQUIRE(h_bins != nullptr, RGN_ERR_INVALID, "null photon bins");
+  const std::vector<float>& y = ftable_y(table);
+  const int T = static_cast<int>(y.size());
+  const double xmin = ftable_xmin(table), xmax = ftable_xmax(table);
+  const double K = (T - 1) / std::log2(xmax / xmin);
+
+  plan.nbins = nbins;
+  plan.bins.assign(h_bins, h_bins + nbins);
+  plan.B0 = B0;
+  plan.g_syn = g_syn;
+  plan.e_at = e_at;
+  plan.K = K;
+  plan.log2_xmin = std::log2(xmin);
+
+  // Zero tail: Z - 1 and Z are zero nodes (or past the end of the table).
+  int last = -1;
+  for (int i = 0; i < T; ++i)
+    if (y[i] != 0.0f) last = i;
+  plan.Z = std::min(last + 2, T);
+
+  std::vector<double> A(nbins, 0.0);
+  std::vector<char> live(nbins, 0);
+  double amin = std::numeric_limits<double>::infinity(), amax = -amin;
+  for (int e = 0; e < nbins; ++e) {
+    const double ee = h_bins[e];
+    if (ee > 0.0 && std::isfinite(ee)) {
+      live[e] = 1;
+      A[e] = (std::log2(ee) - plan.log2_xmin) * K;
+      amin = std::min(amin, A[e]);
+      amax = std::max(amax, A[e]);
+    }
+  }
+  if (!(amin <= amax)) {  // no live bin: every output is 0
+    amin = amax = 0.0;
+  }
+  plan.Amin = amin;
+  plan.Amax = amax;
+  const double span = amax - amin;
+  RGN_REQUIRE(plan.Z + span + 8 < kMaxTableEntries, RGN_ERR_RANGE,
+              "photon bins span too many decades for the LDS table");
+  plan.TP = plan.Z + static_cast<int>(std::ceil(span)) + 4;
+
+  // Padded segment table in u = Z - t.
+  std::vector<float2> tab(plan.TP);
+  for (int j = 0; j < plan.TP; ++j) {
+    const int a = plan.Z - j - 1, b = plan.Z - j;
+    if (a < 0 || b > T - 1) {
+      tab[j] = make_float2(0.0f, 0.0f);
+    } else {
+      const double ya = y[a], yb = y[b];
+      tab[j] = make_float2(static_cast<float>(yb + j * (yb - ya)), static_cast<float>(ya - yb));
+    }
+  }
+  plan.table.reset(plan.TP * sizeof(float2));
+  RGN_HIP(hipMemcpy(plan.table.ptr, tab.data(), plan.table.bytes, hipMemcpyHostToDevice));
+
+  // Per-bin coordinates; dead bins and padding lanes reuse the lowest live A.
+  plan.NB = static_cast<int>(ceil_div(nbins, 64));
+  plan.nbins_pad = 64 * plan.NB;
+  std::vector<double> U(plan.nbins_pad, plan.Z - amin);
+  std::vector<float> bine(plan.nbins_pad, 0.0f);
+  for (int e = 0; e < nbins; ++e) {
+    if (live[e]) {
+      U[e] = plan.Z - A[e];
+      bine[e] = h_bins[e];
+    }
+  }
+  DeviceBuffer dU(plan.nbins_pad * sizeof(double));
+  RGN_HIP(hipMemcpy(dU.ptr, U.data(), dU.bytes, hipMemcpyHostToDevice));
+  plan.bincs.reset(plan.nbins_pad * sizeof(float2));
+  product_pair_kernel<<<plan.nbins_pad, 64>>>(dU.as<double>(), plan.bincs.as<float2>(),
+                                              plan.nbins_pad);
+  RGN_HIP(hipGetLastError());
+  plan.bine.reset(plan.nbins_pad * sizeof(float));
+  RGN_HIP(hipMemcpy(plan.bine.ptr, bine.data(), plan.bine.bytes, hipMemcpyHostToDevice));
+  RGN_HIP(hipDeviceSynchronize());  // dU goes out of scope
+
+  plan.win_lo = amin - plan.Z;
+  plan.win_hi = amax;
+  plan.v_safe = static_cast<float>(std::rint(amax));
+}
+
+PlanDev device_view(const SyncPlan& plan) {
+  PlanDev P{};
+  P.table = plan.table.as<float2>();
+  P.bincs = plan.bincs.as<float2>();
+  P.bine = plan.bine.as<float>();
+  P.TP = plan.TP;
+  P.nbins = plan.nbins;
+  P.nbins_pad = plan.nbins_pad;
+  P.Z = plan.Z;
+  P.B0 = plan.B0;
+  P.g_syn = plan.g_syn;
+  P.e_at = plan.e_at;
+  P.K = plan.K;
+  P.win_lo = plan.win_lo;
+  P.win_hi = plan.win_hi;
+  P.v_safe = plan.v_safe;
+  return P;
+}
+
+size_t forward_smem(const SyncPlan& plan) {
+  const size_t main = kThreads * sizeof(float4) + plan.TP * sizeof(float2);
+  const size_t red = 4 * 64 * plan.NB * sizeof(double);
+  return std::max(main, red);
+}
+
+int forward_grid(size_t n, int blocks_per_cu) {
+  const size_t tiles = ceil_div(n, kThreads);
+  const size_t cap = static_cast<size_t>(num_cus()) * blocks_per_cu;
+  return static_cast<int>(std::max<size_t>(1, std::min(tiles, cap)));
+}
+
+template <class SourceFn>
+void launch_forward(const SyncPlan& plan, SourceFn fn, size_t n, int grid, double* partials,
+                    hipStream_t st) {
+  const PlanDev P = device_view(plan);
+  const size_t smem = forward_smem(plan);
+  switch (plan.NB) {
+#define RGN_FWD_CASE(NB_)                                                          \
+  case NB_:                                                                        \
+    forward_kernel<NB_, SourceFn><<<grid, kThreads, smem, st>>>(fn, n, P, partials); \
+    break;
+    RGN_FWD_CASE(1)
+    RGN_FWD_CASE(2)
+    RGN_FWD_CASE(3)
+    RGN_FWD_CASE(4)
+    RGN_FWD_CASE(5)
+    RGN_FWD_CASE(6)
+    RGN_FWD_CASE(7)
+    RGN_FWD_CASE(8)
+#undef RGN_FWD_CASE
+    default:
+      throw Error(RGN_ERR_UNSUPPORTED, "at most 512 photon bins per plan");
+  }
+  RGN_HIP(hipGetLastError());
+}
+
+void launch_combine(const SyncPlan& plan, const double* partials, int grid, float* out,
+                    hipStream_t st) {
+  combine_kernel<<<plan.nbins, kThreads, 0, st>>>(partials, grid, plan.bine.as<float>(),
+                                                  plan.nbins, out);
+  RGN_HIP(hipGetLastError());
+}
+
+void sync_forward(const SyncPlan& plan, const ParticlesDev& D, size_t n, float* out,
+                  hipStream_t st, int blocks_per_cu) {
+  RGN_REQUIRE(out != nullptr, RGN_ERR_INVALID, "null output");
+  const int grid = forward_grid(n, blocks_per_cu);
+  double* partials =
+    static_cast<double*>(scratch(sizeof(double) * plan.nbins_pad * static_cast<size_t>(grid), 0));
+  launch_forward(plan, ParticleSourceFn{D}, n, grid, partials, st);
+  launch_combine(plan, partials, grid, out, st);
+}
+
+}  // namespace rgn
+
+using namespace rgn;
+
+struct rgn_sync_plan {
+  SyncPlan p;
+};
+
+extern "C" {
+
+int rgn_sync_plan_create(const rgn_ftable* table, const float* h_bins, int nbins, float B0,
+                         float g_syn, float e_at, rgn_sync_plan** out) {
+  return guarded([&] {
+    RGN_REQUIRE(out != nullptr, RGN_ERR_INVALID, "null output handle");
+    RGN_REQUIRE(nbins <= 512, RGN_ERR_UNSUPPORTED, "at most 512 photon bins per plan");
+    auto* plan = new rgn_sync_plan();
+    try {
+      build_sync_plan(plan->p, table, h_bins, nbins, B0, g_syn, e_at);
+    } catch (...) {
+      delete plan;
+      throw;
+    }
+    *out = plan;
+  });
+}
+
+int rgn_sync_plan_destroy(rgn_sync_plan* plan) {
+  delete plan;
+  return RGN_OK;
+}
+
+int rgn_sync_spectrum(const rgn_sync_plan* plan, const float* const d_u[3],
+                      const float* const d_e[3], const float* const d_b[3], const float* d_w,
+                      size_t n, float* d_out, rgn_stream_t stream) {
+  return guarded([&] {
+    RGN_REQUIRE(plan != nullptr, RGN_ERR_INVALID, "null plan");
+    const ParticlesDev D = make_particles(d_u, d_e, d_b, d_w);
+    sync_forward(plan->p, D, n, d_out, as_stream(stream), 4);
+  });
+}
+
+int rgn_sync_adjoint(const rgn_sync_plan* plan, const float* const d_u[3],
+                     const float* const d_e[3], const float* const d_b[3], const float* d_w,
+                     size_t n, const float* d_gbar, float* d_gw, float* const d_gu[3],
+                     float* const d_gb[3], float* const d_ge[3], rgn_stream_t stream) {
+  return guarded([&] {
+    RGN_REQUIRE(plan != nullptr, RGN_ERR_INVALID, "null plan");
+    RGN_REQUIRE(d_gbar != nullptr, RGN_ERR_INVALID, "null upstream gradient");
+    const ParticlesDev D = make_particles(d_u, d_e, d_b, d_w);
+    GradOut G{};
+    G.gw = d_gw;
+    for (int k = 0; k < 3; ++k) {
+      G.gu[k] = d_gu ? d_gu[k] : nullptr;
+      G.gb[k] = d_gb ? d_gb[k] : nullptr;
+      G.ge[k] = d_ge ? d_ge[k] : nullptr;
+    }
+    if (n == 0) return;
+    const SyncPlan& p = plan->p;
+    const PlanDev P = device_view(p);
+    const int grid = forward_grid(n, 4);
+    const size_t smem = kThreads * (sizeof(float4) + sizeof(float2)) + p.TP * sizeof(float2);
+    hipStream_t st = as_stream(stream);
+    switch (p.NB) {
+#define RGN_ADJ_CASE(NB_)                                                              \
+  case NB_:                                                                            \
+    adjoint_kernel<NB_><<<grid, kThreads, smem, st>>>(D, n, P, d_gbar, G);             \
+    break;
+      RGN_ADJ_CASE(1)
+      RGN_ADJ_CASE(2)
+      RGN_ADJ_CASE(3)
+      RGN_ADJ_CASE(4)
+      RGN_ADJ_CASE(5)
+      RGN_ADJ_CASE(6)
+      RGN_ADJ_CASE(7)
+      RGN_ADJ_CASE(8)
+#undef RGN_ADJ_CASE
+      default:
+        throw Error(RGN_ERR_UNSUPPORTED, "at most 512 photon bins per plan");
+    }
+    RGN_HIP(hipGetLastError());
+  });
+}
+
+int rgn_sync_fromdist(const rgn_sync_plan* plan, const float* d_gbeta, const float* d_f, int ng,
+                      int dist_log, float* d_out, rgn_stream_t stream) {
+  return guarded([&] {
+    RGN_REQUIRE(plan != nullptr, RGN_ERR_INVALID, "null plan");
+    RGN_REQUIRE(d_gbeta != nullptr && d_f != nullptr && d_out != nullptr, RGN_ERR_INVALID,
+                "null argument");
+    RGN_REQUIRE(ng >= 0, RGN_ERR_INVALID, "negative distribution size");
+    const SyncPlan& p = plan->p;
+    const int grid = forward_grid(static_cast<size_t>(ng), 4);
+    double* partials =
+      static_cast<double*>(scratch(sizeof(double) * p.nbins_pad * static_cast<size_t>(grid), 1));
+    hipStream_t st = as_stream(stream);
+    launch_forward(p, DistSourceFn{d_gbeta, d_f, dist_log}, static_cast<size_t>(ng), grid,
+                   partials, st);
+    launch_combine(p, partials, grid, d_out, st);
+  });
+}
+
+// Diagnostics: out[5 * n] = (e_peak, chiR, W, V_hi, V_lo) per particle.
+int rgn_sync_particle_terms(const rgn_sync_plan* plan, const float* const d_u[3],
+                            const float* const d_e[3], const float* const d_b[3],
+                            const float* d_w, size_t n, float* d_out, rgn_stream_t stream) {
+  return guarded([&] {
+    RGN_REQUIRE(plan != nullptr, RGN_ERR_INVALID, "null plan");
+    const ParticlesDev D = make_particles(d_u, d_e, d_b, d_w);
+    if (!n) return;
+    particle_terms_kernel<<<static_cast<unsigned>(ceil_div(n, kThreads)), kThreads, 0,
+                            as_stream(stream)>>>(D, n, device_view(plan->p), d_out);
+    RGN_HIP(hipGetLastError());
+  });
+}
+
+// Plan scalars for tests: Z, TP, NB, K, Amin, Amax.
+int rgn_sync_plan_info(const rgn_sync_plan* plan, int* ints3, double* dbl3) {
+  return guarded([&] {
+    RGN_REQUIRE(plan != nullptr, RGN_ERR_INVALID, "null plan");
+    if (ints3) {
+      ints3[0] = plan->p.Z;
+      ints3[1] = plan->p.TP;
+      ints3[2] = plan->p.NB;
+    }
+    if (dbl3) {
+      dbl3[0] = plan->p.K;
+      dbl3[1] = plan->p.Amin;
+      dbl3[2] = plan->p.Amax;
+    }
+  });
+}
+
+}  // extern "C"

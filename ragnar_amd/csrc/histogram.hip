@@ -1,0 +1,235 @@
+// Particles<D>::energyDistribution (particles.cpp:189-260) on gfx950.
+//
+// Index function.  The reference bins an energy E (double) as
+//   E < e_min -> 0,  E >= e_max -> n - 1,
+//   else min(n - 1, (size_t)((float)(n - 1) * |log10(E / e_min)| / log10f(e_max / e_min)))
+// (particles.cpp:233-245), with e_min / e_max the float MinMax of the bins.
+// That function of E is monotone and piecewise constant, so the plan finds its
+// n - 1 jump points E_k (the smallest double E with index >= k) on the host by
+// bisection over the ordered bit patterns of positive doubles, evaluating the
+// reference expression with the host libm.  The kernel takes a float log2
+// estimate of the index and corrects it against E_k and E_{k+1}: the result
+// equals the reference expression bit for bit, whatever the device log does.
+//
+// Accumulation.  Each wave owns a private LDS histogram (uint32 counts and FP64
+// sums of the per-particle float terms); blocks write per-block partials that a
+// second kernel combines in a fixed order.  Counts are exact integers; sums are
+// the FP64 sum of the reference's float terms, rounded to float once.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+#include "rgn_core.hpp"
+
+struct rgn_hist_plan {
+  int nb = 0;
+  int log_bins = 0;
+  float emin = 0.f, emax = 0.f;
+  float approx_scale = 0.f;  // (nb - 1) / log2(emax / emin)
+  std::vector<double> edges;  // nb + 1 entries: edges[k] = E_k for 1 <= k < nb
+  rgn::DeviceBuffer d_edges;
+};
+
+namespace rgn {
+namespace {
+
+constexpr int kThreads = 256;
+
+// particles.cpp:239-243, evaluated exactly as the reference does on the host.
+size_t reference_index(double energy, float emin, float emax, size_t n) {
+  if (energy < emin) return 0;
+  if (energy >= emax) return n - 1;
+  const auto ei = static_cast<std::size_t>(static_cast<float>(n - 1) *
+                                           std::fabs(std::log10(energy / emin)) /
+                                           std::log10(emax / emin));
+  return ei > n - 1 ? n - 1 : ei;
+}
+
+uint64_t bits_of(double d) {
+  uint64_t b;
+  std::memcpy(&b, &d, sizeof b);
+  return b;
+}
+
+double from_bits(uint64_t b) {
+  double d;
+  std::memcpy(&d, &b, sizeof d);
+  return d;
+}
+
+void build_edges(rgn_hist_plan& plan) {
+  const size_t n = plan.nb;
+  plan.edges.assign(n + 1, std::numeric_limits<double>::infinity());
+  plan.edges[0] = -std::numeric_limits<double>::infinity();
+  if (n < 2) return;
+  // All positive doubles in [emin, emax) are ordered like their bit patterns.
+  const uint64_t lo = bits_of(static_cast<double>(plan.emin));
+  const uint64_t hi = bits_of(static_cast<double>(plan.emax));  // exclusive
+  for (size_t k = 1; k < n; ++k) {
+    // smallest E in [emin, emax) with index >= k, or emax if none
+    uint64_t a = lo, b = hi;
+    while (a < b) {
+      const uint64_t mid = a + (b - a) / 2;
+      if (reference_index(from_bits(mid), plan.emin, plan.emax, n) >= k) {
+        b = mid;
+      } else {
+        a = mid + 1;
+      }
+    }
+    // E >= emax maps to n - 1 regardless; an edge at emax covers that.
+    plan.edges[k] = from_bits(a);
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) hist_kernel(const float* __restrict__ ux,
+                                                        const float* __restrict__ uy,
+                                                        const float* __restrict__ uz, size_t n,
+                                                        const double* __restrict__ edges, int nb,
+                                                        float emin, float emax, float scale,
+                                                        int log_bins, int fourvel,
+                                                        unsigned long long* __restrict__ counts,
+                                                        double* __restrict__ partials) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int wave = threadIdx.x >> 6;
+  double* lsum = reinterpret_cast<double*>(smem);                      // [4][nb]
+  unsigned* lcnt = reinterpret_cast<unsigned*>(smem + 4 * nb * sizeof(double));  // [4][nb]
+  for (int i = threadIdx.x; i < 4 * nb; i += kThreads) {
+    lsum[i] = 0.0;
+    lcnt[i] = 0u;
+  }
+  __syncthreads();
+  double* my_sum = lsum + wave * nb;
+  unsigned* my_cnt = lcnt + wave * nb;
+  const size_t stride = static_cast<size_t>(gridDim.x) * kThreads;
+  for (size_t p = blockIdx.x * static_cast<size_t>(kThreads) + threadIdx.x; p < n; p += stride) {
+    const float x = ux[p], y = uy[p], z = uz[p];
+    const float usqr = x * x + y * y + z * z;  // particles.cpp:228-230, float
+    const double energy = fourvel ? static_cast<double>(sqrtf(usqr))
+                                  : sqrt(1.0 + static_cast<double>(usqr));
+    int idx;
+    if (energy < emin) {
+      idx = 0;
+    } else if (energy >= emax) {
+      idx = nb - 1;
+    } else if (energy == energy) {
+      const float est = scale * log2f(static_cast<float>(energy) / emin);
+      idx = static_cast<int>(fminf(fmaxf(est, 0.0f), static_cast<float>(nb - 1)));
+      while (idx < nb - 1 && energy >= edges[idx + 1]) ++idx;
+      while (idx > 0 && energy < edges[idx]) --idx;
+    } else {
+      idx = nb - 1;  // NaN energy: the x86 reference's size_t conversion lands here
+    }
+    const float term = log_bins ? static_cast<float>(1.0 / energy) : 1.0f;
+    atomicAdd(my_cnt + idx, 1u);
+    atomicAdd(my_sum + idx, static_cast<double>(term));
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nb; i += kThreads) {
+    const unsigned c = lcnt[i] + lcnt[nb + i] + lcnt[2 * nb + i] + lcnt[3 * nb + i];
+    const double s = ((lsum[i] + lsum[nb + i]) + lsum[2 * nb + i]) + lsum[3 * nb + i];
+    if (counts && c) atomicAdd(counts + i, static_cast<unsigned long long>(c));
+    partials[static_cast<size_t>(i) * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) hist_combine_kernel(const double* __restrict__ partials,
+                                                                int nblocks, int nb,
+                                                                float* __restrict__ out) {
+  const int i = blockIdx.x;
+  if (i >= nb) return;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += kThreads) s += partials[static_cast<size_t>(i) * nblocks + b];
+  __shared__ double red[kThreads];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (static_cast<int>(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[i] = static_cast<float>(red[0]);
+}
+
+}  // namespace
+}  // namespace rgn
+
+using namespace rgn;
+
+extern "C" {
+
+int rgn_hist_plan_create(const float* h_bins, int nb, int log_bins, rgn_hist_plan** out) {
+  return guarded([&] {
+    RGN_REQUIRE(out != nullptr && h_bins != nullptr, RGN_ERR_INVALID, "null argument");
+    RGN_REQUIRE(nb >= 1, RGN_ERR_INVALID, "no energy bins");
+    RGN_REQUIRE(static_cast<size_t>(nb) * 4 * (sizeof(double) + sizeof(unsigned)) <= 150 * 1024,
+                RGN_ERR_UNSUPPORTED, "too many energy bins for the LDS histogram");
+    auto* plan = new rgn_hist_plan();
+    plan->nb = nb;
+    plan->log_bins = log_bins ? 1 : 0;
+    // particles.cpp:200-215 ("EnergyBinsMinMax")
+    float lo = h_bins[0], hi = h_bins[0];
+    for (int i = 0; i < nb; ++i) {
+      lo = h_bins[i] < lo ? h_bins[i] : lo;
+      hi = h_bins[i] > hi ? h_bins[i] : hi;
+    }
+    plan->emin = lo;
+    plan->emax = hi;
+    plan->approx_scale =
+      (lo > 0.0f && hi > lo) ? static_cast<float>((nb - 1) / std::log2(static_cast<double>(hi) / lo))
+                             : 0.0f;
+    try {
+      if (lo > 0.0f && hi > lo) {
+        build_edges(*plan);
+      } else {
+        // degenerate bins: every finite energy lands in bin 0 or n - 1
+        plan->edges.assign(nb + 1, std::numeric_limits<double>::infinity());
+        plan->edges[0] = -std::numeric_limits<double>::infinity();
+      }
+      plan->d_edges.reset(plan->edges.size() * sizeof(double));
+      RGN_HIP(hipMemcpy(plan->d_edges.ptr, plan->edges.data(), plan->d_edges.bytes,
+                        hipMemcpyHostToDevice));
+    } catch (...) {
+      delete plan;
+      throw;
+    }
+    *out = plan;
+  });
+}
+
+int rgn_hist_plan_destroy(rgn_hist_plan* plan) {
+  delete plan;
+  return RGN_OK;
+}
+
+int rgn_energy_hist(const rgn_hist_plan* plan, const float* const d_u[3], size_t n, int fourvel,
+                    float* d_hist, uint64_t* d_counts, rgn_stream_t stream) {
+  return guarded([&] {
+    RGN_REQUIRE(plan != nullptr, RGN_ERR_INVALID, "null plan");
+    RGN_REQUIRE(d_u && d_u[0] && d_u[1] && d_u[2] && d_hist, RGN_ERR_INVALID, "null argument");
+    hipStream_t st = as_stream(stream);
+    const int nb = plan->nb;
+    const size_t tiles = ceil_div(std::max<size_t>(n, 1), kThreads);
+    const int grid = static_cast<int>(std::min<size_t>(tiles, static_cast<size_t>(num_cus()) * 4));
+    double* partials = static_cast<double*>(scratch(sizeof(double) * nb * static_cast<size_t>(grid), 2));
+    if (d_counts) RGN_HIP(hipMemsetAsync(d_counts, 0, sizeof(uint64_t) * nb, st));
+    const size_t smem = 4 * nb * (sizeof(double) + sizeof(unsigned));
+    hist_kernel<<<grid, kThreads, smem, st>>>(
+      d_u[0], d_u[1], d_u[2], n, plan->d_edges.as<double>(), nb, plan->emin, plan->emax,
+      plan->approx_scale, plan->log_bins, fourvel ? 1 : 0,
+      reinterpret_cast<unsigned long long*>(d_counts), partials);
+    RGN_HIP(hipGetLastError());
+    hist_combine_kernel<<<nb, kThreads, 0, st>>>(partials, grid, nb, d_hist);
+    RGN_HIP(hipGetLastError());
+  });
+}
+
+// Jump points of the index function (nb + 1 doubles), for tests.
+int rgn_hist_plan_edges(const rgn_hist_plan* plan, double* h_edges) {
+  return guarded([&] {
+    RGN_REQUIRE(plan != nullptr && h_edges != nullptr, RGN_ERR_INVALID, "null argument");
+    std::copy(plan->edges.begin(), plan->edges.end(), h_edges);
+  });
+}
+
+}  // extern "C"

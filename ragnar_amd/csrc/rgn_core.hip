@@ -1,0 +1,240 @@
+// Runtime entry points of libragnar_hip.so: init, errors, device buffers and
+// the per-device scratch cache.
+#include "rgn_core.hpp"
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace rgn {
+
+namespace {
+thread_local std::string g_last_error;
+std::mutex g_init_mutex;
+bool g_initialized = false;
+int g_device = 0;
+int g_num_cus = 0;
+
+struct Scratch {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+std::mutex g_scratch_mutex;
+std::vector<Scratch> g_scratch;  // indexed by slot
+}  // namespace
+
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+[[noreturn]] void throw_hip(hipError_t e, const char* what, const char* file, int line) {
+  const int code = (e == hipErrorOutOfMemory) ? RGN_ERR_NOMEM : RGN_ERR_HIP;
+  throw Error(code, std::string("HIP error '") + hipGetErrorString(e) + "' in " + what +
+                      " at " + file + ":" + std::to_string(line));
+}
+
+static void do_init(int device) {
+  int count = 0;
+  RGN_HIP(hipGetDeviceCount(&count));
+  RGN_REQUIRE(count > 0, RGN_ERR_HIP, "no HIP device visible");
+  RGN_REQUIRE(device >= 0 && device < count, RGN_ERR_INVALID,
+              "device index " + std::to_string(device) + " out of range");
+  RGN_HIP(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  RGN_HIP(hipGetDeviceProperties(&prop, device));
+  const std::string arch = prop.gcnArchName;
+  RGN_REQUIRE(arch.rfind("gfx950", 0) == 0, RGN_ERR_UNSUPPORTED,
+              "libragnar_hip is built for gfx950; device " + std::to_string(device) +
+                " is " + arch);
+  g_device = device;
+  g_num_cus = prop.multiProcessorCount;
+  g_initialized = true;
+}
+
+void ensure_init() {
+  std::lock_guard<std::mutex> lk(g_init_mutex);
+  if (!g_initialized) do_init(0);
+}
+
+int num_cus() { return g_num_cus > 0 ? g_num_cus : 256; }
+
+void DeviceBuffer::reset(size_t n) {
+  release();
+  if (n == 0) return;
+  RGN_HIP(hipMalloc(&ptr, n));
+  bytes = n;
+}
+
+void DeviceBuffer::release() {
+  if (ptr) {
+    (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+}
+
+void* scratch(size_t bytes, int slot) {
+  std::lock_guard<std::mutex> lk(g_scratch_mutex);
+  if (static_cast<size_t>(slot) >= g_scratch.size()) g_scratch.resize(slot + 1);
+  Scratch& s = g_scratch[slot];
+  if (s.bytes < bytes) {
+    if (s.ptr) {
+      // the previous contents may still be read by queued work
+      RGN_HIP(hipDeviceSynchronize());
+      RGN_HIP(hipFree(s.ptr));
+      s.ptr = nullptr;
+      s.bytes = 0;
+    }
+    const size_t grow = bytes + bytes / 4 + 4096;
+    RGN_HIP(hipMalloc(&s.ptr, grow));
+    s.bytes = grow;
+  }
+  return s.ptr;
+}
+
+}  // namespace rgn
+
+using namespace rgn;
+
+extern "C" {
+
+int rgn_init(int device) {
+  try {
+    std::lock_guard<std::mutex> lk(g_init_mutex);
+    if (g_initialized) {
+      RGN_REQUIRE(device == g_device, RGN_ERR_INVALID,
+                  "already initialised on device " + std::to_string(g_device));
+      return RGN_OK;
+    }
+    do_init(device);
+    return RGN_OK;
+  } catch (const Error& e) {
+    set_last_error(e.what());
+    return e.code;
+  }
+}
+
+int rgn_finalize(void) {
+  try {
+    std::lock_guard<std::mutex> lk(g_init_mutex);
+    if (!g_initialized) return RGN_OK;
+    RGN_HIP(hipDeviceSynchronize());
+    {
+      std::lock_guard<std::mutex> lk2(g_scratch_mutex);
+      for (auto& s : g_scratch) {
+        if (s.ptr) (void)hipFree(s.ptr);
+        s = Scratch{};
+      }
+    }
+    g_initialized = false;
+    return RGN_OK;
+  } catch (const Error& e) {
+    set_last_error(e.what());
+    return e.code;
+  }
+}
+
+int rgn_is_initialized(void) {
+  std::lock_guard<std::mutex> lk(g_init_mutex);
+  return g_initialized ? 1 : 0;
+}
+
+const char* rgn_last_error(void) { return g_last_error.c_str(); }
+
+const char* rgn_version(void) { return "ragnar_hip 0.1 (gfx950)"; }
+
+int rgn_sync(rgn_stream_t stream) {
+  return guarded([&] { RGN_HIP(hipStreamSynchronize(as_stream(stream))); });
+}
+
+int rgn_malloc(void** d_ptr, size_t bytes) {
+  return guarded([&] {
+    *d_ptr = nullptr;
+    if (bytes) RGN_HIP(hipMalloc(d_ptr, bytes));
+  });
+}
+
+int rgn_free(void* d_ptr) {
+  return guarded([&] {
+    if (d_ptr) RGN_HIP(hipFree(d_ptr));
+  });
+}
+
+int rgn_memcpy_h2d(void* d_dst, const void* h_src, size_t bytes) {
+  return guarded([&] {
+    if (bytes) RGN_HIP(hipMemcpy(d_dst, h_src, bytes, hipMemcpyHostToDevice));
+  });
+}
+
+int rgn_memcpy_d2h(void* h_dst, const void* d_src, size_t bytes) {
+  return guarded([&] {
+    if (bytes) RGN_HIP(hipMemcpy(h_dst, d_src, bytes, hipMemcpyDeviceToHost));
+  });
+}
+
+int rgn_memcpy_d2d(void* d_dst, const void* d_src, size_t bytes) {
+  return guarded([&] {
+    if (bytes) RGN_HIP(hipMemcpy(d_dst, d_src, bytes, hipMemcpyDeviceToDevice));
+  });
+}
+
+int rgn_memset(void* d_dst, int value, size_t bytes) {
+  return guarded([&] {
+    if (bytes) RGN_HIP(hipMemset(d_dst, value, bytes));
+  });
+}
+
+int rgn_host_malloc(void** h_ptr, size_t bytes) {
+  return guarded([&] {
+    *h_ptr = nullptr;
+    if (bytes) RGN_HIP(hipHostMalloc(h_ptr, bytes, hipHostMallocDefault));
+  });
+}
+
+int rgn_host_free(void* h_ptr) {
+  return guarded([&] {
+    if (h_ptr) RGN_HIP(hipHostFree(h_ptr));
+  });
+}
+
+int rgn_memcpy_h2d_async(void* d_dst, const void* h_src, size_t bytes, rgn_stream_t stream) {
+  return guarded([&] {
+    if (bytes) RGN_HIP(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, as_stream(stream)));
+  });
+}
+
+int rgn_stream_create(rgn_stream_t* stream) {
+  return guarded([&] {
+    hipStream_t s = nullptr;
+    RGN_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = s;
+  });
+}
+
+int rgn_stream_destroy(rgn_stream_t stream) {
+  return guarded([&] {
+    if (stream) RGN_HIP(hipStreamDestroy(as_stream(stream)));
+  });
+}
+
+int rgn_event_create(void** event) {
+  return guarded([&] {
+    hipEvent_t e = nullptr;
+    RGN_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    *event = e;
+  });
+}
+
+int rgn_event_destroy(void* event) {
+  return guarded([&] {
+    if (event) RGN_HIP(hipEventDestroy(static_cast<hipEvent_t>(event)));
+  });
+}
+
+int rgn_event_record(void* event, rgn_stream_t stream) {
+  return guarded([&] { RGN_HIP(hipEventRecord(static_cast<hipEvent_t>(event), as_stream(stream))); });
+}
+
+int rgn_event_sync(void* event) {
+  return guarded([&] { RGN_HIP(hipEventSynchronize(static_cast<hipEvent_t>(event))); });
+}
+
+}  // extern "C"

@@ -1,0 +1,85 @@
+// Internal helpers shared by the HIP translation units of libragnar_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "ragnar_hip.h"
+
+namespace rgn {
+
+// An error carrying the C-ABI status code.
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& msg) : std::runtime_error(msg), code(c) {}
+};
+
+void set_last_error(const std::string& msg);
+void ensure_init();
+
+[[noreturn]] void throw_hip(hipError_t e, const char* what, const char* file, int line);
+
+#define RGN_HIP(call)                                          \
+  do {                                                         \
+    hipError_t rgn_e_ = (call);                                \
+    if (rgn_e_ != hipSuccess) {                                \
+      ::rgn::throw_hip(rgn_e_, #call, __FILE__, __LINE__);     \
+    }                                                          \
+  } while (0)
+
+#define RGN_REQUIRE(cond, code, msg)                           \
+  do {                                                         \
+    if (!(cond)) throw ::rgn::Error((code), (msg));            \
+  } while (0)
+
+// Runs f() and converts any exception to a status code + last-error message.
+template <class F>
+int guarded(F&& f) {
+  try {
+    ensure_init();
+    f();
+    return RGN_OK;
+  } catch (const Error& e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc& e) {
+    set_last_error(std::string("host allocation failed: ") + e.what());
+    return RGN_ERR_NOMEM;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    return RGN_ERR_INVALID;
+  }
+}
+
+// A device buffer owned by the library (plans, tables).
+struct DeviceBuffer {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  DeviceBuffer() = default;
+  explicit DeviceBuffer(size_t n) { reset(n); }
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+  ~DeviceBuffer() { release(); }
+  void reset(size_t n);
+  void release();
+  template <class T>
+  T* as() const { return static_cast<T*>(ptr); }
+};
+
+// Per-device scratch reused across calls on the same stream order (grows on
+// demand; never shrinks).  Calls that overlap on different streams must not
+// share it; the library serialises scratch users through `scratch_lock`.
+void* scratch(size_t bytes, int slot);
+
+// Number of compute units of the current device.
+int num_cus();
+
+inline hipStream_t as_stream(rgn_stream_t s) { return static_cast<hipStream_t>(s); }
+
+inline size_t ceil_div(size_t a, size_t b) { return (a + b - 1) / b; }
+
+}  // namespace rgn

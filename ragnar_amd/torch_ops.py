@@ -1,0 +1,181 @@
+"""Torch-facing wrappers of the C ABI: device tensors in, device tensors out.
+
+Tensors are PyTorch-ROCm ``cuda`` tensors used only as HBM buffers and
+streams; every computation runs in the HIP kernels of libragnar_hip.so on the
+current torch stream.  Particle data is structure-of-arrays: ``u``, ``b`` and
+``e`` are (3, N) float32 tensors (one contiguous row per component).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+from . import _capi
+from ._capi import FTable, HistPlan, SyncPlan, check, lib, ptr3
+
+__all__ = ["FTable", "SyncPlan", "HistPlan", "sync_spectrum", "sync_adjoint", "sync_cells",
+           "sync_fromdist", "sync_fromdist_batched", "energy_hist", "particle_terms",
+           "SynchrotronSpectrumFn", "synchrotron_spectrum"]
+
+_default_table: FTable | None = None
+
+
+def default_table() -> FTable:
+    """The reference F table (TabulateFfunc defaults), built once."""
+    global _default_table
+    if _default_table is None:
+        _default_table = FTable(_capi.RGN_FTABLE_REFERENCE, 200, 1e-6, 100.0)
+    return _default_table
+
+
+def _stream() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _soa(t: torch.Tensor | None, name: str, n: int | None = None):
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a device tensor (the kernels run on the GPU only)")
+    if t.dtype != torch.float32:
+        raise ValueError(f"{name} must be float32")
+    if t.dim() != 2 or t.shape[0] != 3:
+        raise ValueError(f"{name} must have shape (3, N)")
+    if n is not None and t.shape[1] != n:
+        raise ValueError(f"{name} has {t.shape[1]} particles, expected {n}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    return ptr3([t[k].data_ptr() for k in range(3)])
+
+
+def _vec(t: torch.Tensor | None, name: str, n: int):
+    if t is None:
+        return None
+    if not t.is_cuda or t.dtype != torch.float32 or t.dim() != 1 or t.shape[0] != n:
+        raise ValueError(f"{name} must be a float32 device vector of length {n}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    return t.data_ptr()
+
+
+def sync_spectrum(plan: SyncPlan, u, b, w=None, e=None, out=None) -> torch.Tensor:
+    n = u.shape[1]
+    if out is None:
+        out = torch.empty(plan.nbins, dtype=torch.float32, device=u.device)
+    check(lib().rgn_sync_spectrum(plan.handle, _soa(u, "u"), _soa(e, "e", n) or ptr3(None, None, None),
+                                  _soa(b, "b", n), _vec(w, "w", n), n, out.data_ptr(), _stream()))
+    return out
+
+
+def sync_adjoint(plan: SyncPlan, u, b, w, e, gbar, need=("w", "u", "b", "e")):
+    n = u.shape[1]
+    dev = u.device
+    gw = torch.empty(n, dtype=torch.float32, device=dev) if "w" in need else None
+    gu = torch.empty(3, n, dtype=torch.float32, device=dev) if "u" in need else None
+    gb = torch.empty(3, n, dtype=torch.float32, device=dev) if "b" in need else None
+    ge = torch.empty(3, n, dtype=torch.float32, device=dev) if "e" in need else None
+    gbar = gbar.contiguous().float()
+    none3 = ptr3(None, None, None)
+    check(lib().rgn_sync_adjoint(
+        plan.handle, _soa(u, "u"), _soa(e, "e", n) or none3, _soa(b, "b", n), _vec(w, "w", n), n,
+        _vec(gbar, "gbar", plan.nbins), gw.data_ptr() if gw is not None else None,
+        _soa(gu, "gu", n) or none3, _soa(gb, "gb", n) or none3, _soa(ge, "ge", n) or none3,
+        _stream()))
+    return gw, gu, gb, ge
+
+
+def sync_cells(plan: SyncPlan, u, b, offsets, w=None, e=None, out=None) -> torch.Tensor:
+    """Spectra of ncells particle slices; offsets: host int64 array of ncells + 1."""
+    import numpy as np
+
+    offs = np.ascontiguousarray(offsets, np.int64)
+    ncells = len(offs) - 1
+    n = u.shape[1]
+    if offs[-1] > n:
+        raise ValueError("cell offsets exceed the particle count")
+    if out is None:
+        out = torch.empty(ncells, plan.nbins, dtype=torch.float32, device=u.device)
+    check(lib().rgn_sync_cells(plan.handle, _soa(u, "u"), _soa(e, "e", n) or ptr3(None, None, None),
+                               _soa(b, "b", n), _vec(w, "w", n), offs.ctypes.data, ncells,
+                               out.data_ptr(), _stream()))
+    return out
+
+
+def sync_fromdist(plan: SyncPlan, gbeta, f, dist_log: bool, out=None) -> torch.Tensor:
+    ng = gbeta.shape[0]
+    if out is None:
+        out = torch.empty(plan.nbins, dtype=torch.float32, device=gbeta.device)
+    check(lib().rgn_sync_fromdist(plan.handle, _vec(gbeta, "gbeta", ng), _vec(f, "f", ng), ng,
+                                  int(dist_log), out.data_ptr(), _stream()))
+    return out
+
+
+def sync_fromdist_batched(plan: SyncPlan, gbeta, f2d, dist_log: bool, precision: int,
+                          out=None) -> torch.Tensor:
+    ng = gbeta.shape[0]
+    nd = f2d.shape[0]
+    if f2d.shape[1] != ng or not f2d.is_contiguous() or f2d.dtype != torch.float32:
+        raise ValueError("f must be a contiguous float32 (nd, ng) tensor")
+    if out is None:
+        out = torch.empty(nd, plan.nbins, dtype=torch.float32, device=gbeta.device)
+    check(lib().rgn_sync_fromdist_batched(plan.handle, _vec(gbeta, "gbeta", ng), ng, int(dist_log),
+                                          f2d.data_ptr(), nd, precision, out.data_ptr(), _stream()))
+    return out
+
+
+def energy_hist(plan: HistPlan, u, fourvel: bool = True, counts: bool = True):
+    n = u.shape[1]
+    hist = torch.empty(plan.nb, dtype=torch.float32, device=u.device)
+    cnt = torch.empty(plan.nb, dtype=torch.int64, device=u.device) if counts else None
+    check(lib().rgn_energy_hist(plan.handle, _soa(u, "u"), n, int(fourvel), hist.data_ptr(),
+                                cnt.data_ptr() if cnt is not None else None, _stream()))
+    return hist, cnt
+
+
+def particle_terms(plan: SyncPlan, u, b, w=None, e=None) -> torch.Tensor:
+    n = u.shape[1]
+    out = torch.empty(n, 5, dtype=torch.float32, device=u.device)
+    check(lib().rgn_sync_particle_terms(plan.handle, _soa(u, "u"), _soa(e, "e", n) or ptr3(None, None, None),
+                                        _soa(b, "b", n), _vec(w, "w", n), n, out.data_ptr(),
+                                        _stream()))
+    return out
+
+
+class SynchrotronSpectrumFn(torch.autograd.Function):
+    """S = SynchrotronSpectrum(u, b, w[, e]) with the hand-written adjoint.
+
+    Forward: the gfx950 forward kernel.  Backward: the adjoint kernel, which
+    recomputes gamma and the pitch-angle terms of each particle instead of
+    storing them; gradients use the table's segment slopes (the exact adjoint
+    of the piecewise-linear interpolant that the forward evaluates).
+    """
+
+    @staticmethod
+    def forward(ctx, u, b, w, e, plan):
+        out = sync_spectrum(plan, u.detach(), b.detach(), None if w is None else w.detach(),
+                            None if e is None else e.detach())
+        ctx.plan = plan
+        ctx.save_for_backward(u, b, w, e)
+        return out
+
+    @staticmethod
+    def backward(ctx, gout):
+        u, b, w, e = ctx.saved_tensors
+        need = []
+        if ctx.needs_input_grad[0]:
+            need.append("u")
+        if ctx.needs_input_grad[1]:
+            need.append("b")
+        if ctx.needs_input_grad[2]:
+            need.append("w")
+        if ctx.needs_input_grad[3]:
+            need.append("e")
+        gw, gu, gb, ge = sync_adjoint(ctx.plan, u.detach(), b.detach(),
+                                      None if w is None else w.detach(),
+                                      None if e is None else e.detach(), gout, need=tuple(need))
+        return gu, gb, gw, ge, None
+
+
+def synchrotron_spectrum(u, b, w=None, e=None, *, plan: SyncPlan):
+    return SynchrotronSpectrumFn.apply(u, b, w, e, plan)
