@@ -35,15 +35,58 @@ HIST_BYTES_PER_PARTICLE = 12
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--profile-reps", type=int, default=10)
     ap.add_argument("--particles", type=int, default=1 << 24)
     ap.add_argument("--bins", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="only the headline step")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU work (thread-seconds) of the baseline sample")
+    ap.add_argument("--profile", action="store_true",
+                    help="dispatch each full-size kernel once (forward, adjoint, histogram) and a "
+                         "1 GiB device copy that calibrates rocprofv3 byte counters, then exit")
     return ap.parse_args()
+
+
+def profile_run(args):
+    """Single dispatches for rocprofv3 --pmc / --kernel-trace runs."""
+    import numpy as np
+    import torch
+
+    from ragnar_amd import torch_ops as T
+    from ragnar_amd import workloads
+    from oracle import oracle as orc
+
+    n, nb = args.particles, args.bins
+    u_h, b_h = workloads.particles(n)
+    U = torch.from_numpy(u_h).cuda()
+    B = torch.from_numpy(b_h).cuda()
+    plan = T.SyncPlan(T.default_table(), orc.logspace(1e-2, 1e7, nb))
+    hplan = T.HistPlan(orc.logspace(1e-2, 1e3, 200), True)
+    src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
+    dst = torch.empty_like(src)
+    src.fill_(1.0)
+    S = torch.empty(nb, device="cuda")
+    ones = torch.ones(nb, device="cuda")
+    reps = args.profile_reps
+    for _ in range(args.warmup):  # clocks ramp up before the counted dispatches
+        T.sync_spectrum(plan, U, B, out=S)
+    torch.cuda.synchronize()
+    print(f"profile: forward x{reps}", flush=True)
+    for _ in range(reps):
+        T.sync_spectrum(plan, U, B, out=S)
+    print(f"profile: energy histogram x{reps}", flush=True)
+    for _ in range(reps):
+        T.energy_hist(hplan, U, True)
+    print(f"profile: adjoint x{reps}", flush=True)
+    for _ in range(reps):
+        T.sync_adjoint(plan, U, B, None, None, ones, need=("u", "b", "w"))
+    print("profile: 1 GiB copy (calibration)", flush=True)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print("profile: done", flush=True)
 
 
 def event_time(fn, steps, warmup, torch):
@@ -62,15 +105,74 @@ def event_time(fn, steps, warmup, torch):
     return t0.elapsed_time(t1) / steps
 
 
-def read_traffic(name):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
-    p = ROOT / "profiles" / f"{name}.traffic.json"
+def read_traffic(kernel):
+    """HBM bytes per launch of a kernel at the benchmark size, from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE, each scaled by what the
+    counter reads for a 1 GiB device copy), or None."""
+    p = ROOT / "profiles" / "traffic_2p24.json"
     if p.exists():
         try:
-            return json.loads(p.read_text()).get("bytes_per_launch")
+            return json.loads(p.read_text())[kernel]["bytes_per_launch"]
         except Exception:
             return None
     return None
+
+
+def bench_cells(T, workloads, orc, torch, table):
+    """4096 cells, lognormal counts (median 4000, sigma 1): one persistent launch
+    in a hipGraph against 4096 calls of the single-population form."""
+    import numpy as np
+
+    u, b, offs, _, _ = workloads.cells(4096, 4000, 1.0)
+    nb = 256
+    plan = T.SyncPlan(table, orc.logspace(1e-2, 1e7, nb))
+    U = torch.from_numpy(u).cuda()
+    B = torch.from_numpy(b).cuda()
+    out = torch.empty(4096, nb, device="cuda")
+    ms_graph = event_time(lambda: T.sync_cells(plan, U, B, offs, out=out), 10, 2, torch)
+    views = [(U[:, a:z], B[:, a:z]) for a, z in zip(offs[:-1], offs[1:])]
+    rows = [out[c] for c in range(4096)]
+
+    def per_cell():
+        for (uc, bc), o in zip(views, rows):
+            T.sync_spectrum(plan, uc.contiguous() if not uc.is_contiguous() else uc, bc, out=o)
+
+    # views of a (3, N) tensor are not contiguous: copy them once, outside the timing
+    views = [(uc.contiguous(), bc.contiguous()) for uc, bc in views]
+    ms_calls = event_time(per_cell, 2, 1, torch)
+    total = int(offs[-1])
+    return {"particles": total, "ms_graph": ms_graph, "ms_per_cell_calls": ms_calls,
+            "speedup": ms_calls / ms_graph, "evals_per_s": total * nb / (ms_graph * 1e-3),
+            "frac_fp32_peak": 6 * total * nb / (ms_graph * 1e-3) / PEAK_FP32}
+
+
+def bench_contraction(T, workloads, orc, torch, table):
+    """65536 power-law x cutoff distributions on 512 log gamma bins -> 512 photon
+    bins, FP32 GEMM on the matrix cores; error against the FP64 oracle on 32 rows."""
+    import numpy as np
+
+    nd, ng, nb = 65536, 512, 512
+    f, _, _ = workloads.distributions(nd, ng)
+    g = orc.logspace(1, 1e6, ng)
+    bins = orc.logspace(1e-2, 1e7, nb)
+    plan = T.SyncPlan(table, bins)
+    G = torch.from_numpy(g).cuda()
+    Fm = torch.from_numpy(f).cuda()
+    out = torch.empty(nd, nb, device="cuda")
+    res = {}
+    for name, prec in (("fp32", 0),):
+        ms = event_time(lambda: T.sync_fromdist_batched(plan, G, Fm, True, prec, out=out), 10, 2, torch)
+        S = out.cpu().numpy()
+        worst = 0.0
+        for d in range(0, nd, nd // 32):
+            _, f64 = orc.sync_fromdist(g, f[d], True, bins, mode=orc.FP64, want_fp64=True)
+            live = f64 > 1e-12 * f64.max()
+            worst = max(worst, float(np.max(np.abs(S[d][live] / f64[live] - 1))))
+        flop = 2.0 * nd * ng * nb
+        res[name] = {"ms": ms, "TFLOP_per_s": flop / (ms * 1e-3) / 1e12,
+                     "GB_per_s": (nd * ng + nd * nb) * 4 / (ms * 1e-3) / 1e9,
+                     "max_rel_err_vs_fp64_32rows": worst}
+    return res
 
 
 def cpu_baseline(u, b, bins, target_cpu_seconds):
@@ -102,6 +204,10 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    "(multi-GPU sharding is described in DESIGN.md, not launched)"}))
         sys.exit(2)
+
+    if args.profile:
+        profile_run(args)
+        return
 
     import numpy as np
     import torch
@@ -155,7 +261,8 @@ def main():
                    "particles": n, "photon_bins": nb},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
-                     "traffic": read_traffic("sync_forward_2p24x256"),
+                     "traffic": read_traffic("forward_kernel"),
+                     "algorithmic_bytes": 24 * n,
                      "kernel": "forward_kernel (+ the fixed-order combine of the same step)",
                      "flop_per_eval": FLOP_PER_EVAL_FWD},
         "parity_max_rel_err_subset": parity,
@@ -192,6 +299,13 @@ def main():
         G, Fd = torch.from_numpy(g).to(dev), torch.from_numpy(f).to(dev)
         ms_d = event_time(lambda: T.sync_fromdist(dplan, G, Fd, True), 50, 5, torch)
         extras["fromdist_200x200"] = {"us": ms_d * 1e3}
+        for name, fn in (("cells_4096", lambda: bench_cells(T, workloads, orc, torch, table)),
+                         ("contraction_65536x512x512",
+                          lambda: bench_contraction(T, workloads, orc, torch, table))):
+            try:
+                extras[name] = fn()
+            except Exception as exc:  # reported, the headline line still prints
+                extras[name] = {"error": repr(exc)}
     res["extras"] = extras
 
     if not args.no_cpu:

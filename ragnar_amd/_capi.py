@@ -110,9 +110,12 @@ def declared_symbols() -> list[str]:
     return sorted(set(re.findall(r"\b(rgn_[a-z0-9_]+)\s*\(", text)))
 
 
-def ptr3(a, b=None, c=None):
-    """Three device addresses (ints or None) as a C array of void*."""
-    vals = list(a) if b is None and c is None and not isinstance(a, int) else [a, b, c]
+def ptr3(*args):
+    """Three device addresses (ints or None), given as three arguments or one
+    sequence, as a C array of void*."""
+    vals = list(args[0]) if len(args) == 1 else list(args)
+    if len(vals) != 3:
+        raise ValueError("ptr3 needs three addresses")
     return fptr3(*[v if v else None for v in vals])
 
 

@@ -25,7 +25,8 @@ CSRC = HERE / "csrc"
 BUILD = HERE / "_build"
 INCLUDE = ROOT / "include"
 
-HIP_SOURCES = ["rgn_core.hip", "ftable.hip", "emission.hip", "histogram.hip"]
+HIP_SOURCES = ["rgn_core.hip", "ftable.hip", "emission.hip", "cells.hip", "histogram.hip",
+               "contraction.hip"]
 HOST_SOURCES = ["host/pyragnar.cpp", "host/containers.cpp", "host/h5io.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -85,7 +86,7 @@ def build(jobs: int | None = None, verbose: bool = False) -> None:
     lib = library_path()
     tmp = lib.with_suffix(".so.tmp")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp),
-          *map(str, hip_objs)])
+          *map(str, hip_objs), "-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, lib)
 
     mod = module_path()

@@ -1,0 +1,99 @@
+// Batched distribution -> spectrum contraction S[d, e] = sum_g f[d, g] K[g, e]
+// for nd distributions on one gbeta grid: KernelFromDist (synchrotron.hpp:72-95)
+// applied to every row, written as a GEMM.
+//
+// K[g, e] = e_e * (gbeta_g if the distribution bins are log spaced) *
+// F(e_e / eps_g), eps_g = e_at gbeta_g^2 / g_syn^2, is built on the GPU with the
+// same interpolation coordinate and segment table as the emission kernels
+// (one thread per entry).  The GEMM then runs on the matrix cores:
+//   RGN_PREC_FP32  rocBLAS SGEMM (a plain library GEMM, FP32 throughout);
+//   RGN_PREC_F16X3 / RGN_PREC_F16X1: split-FP16 MFMA kernel (contraction_f16.hip).
+#include <rocblas/rocblas.h>
+
+#include <mutex>
+
+#include "emission.hpp"
+#include "rgn_core.hpp"
+
+namespace rgn {
+
+void contraction_f16(const SyncPlan&, const float*, int, const float*, size_t, int, float*,
+                     hipStream_t) {
+  throw Error(RGN_ERR_UNSUPPORTED, "split-FP16 contraction is not available in this build");
+}
+
+namespace {
+
+__global__ void __launch_bounds__(256) kernel_matrix(PlanDev P, const float* __restrict__ gbeta,
+                                                     int ng, int dist_log, float* __restrict__ K) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  const int g = blockIdx.y;
+  if (e >= P.nbins) return;
+  const float gb = gbeta[g];
+  const float eps = P.e_at * gb * gb / (P.g_syn * P.g_syn);  // synchrotron.hpp:78-79
+  const Source s = make_source(dist_log ? gb : 1.0f, eps, P);
+  float v = 0.0f;
+  if (s.W != 0.0f) {
+    const float2 cs_e = P.bincs[e];
+    const float u = fmaf(cs_e.x, cs_e.y, s.vhi) + s.vlo;
+    const float2 cs = P.table[cvt_sat_u32(u)];
+    v = P.bine[e] * (s.W * fmaf(u, cs.y, cs.x));
+  }
+  K[static_cast<size_t>(g) * P.nbins + e] = v;
+}
+
+std::mutex g_blas_mutex;
+rocblas_handle g_blas = nullptr;
+
+rocblas_handle blas() {
+  if (!g_blas) {
+    if (rocblas_create_handle(&g_blas) != rocblas_status_success) {
+      throw Error(RGN_ERR_HIP, "rocblas_create_handle failed");
+    }
+  }
+  return g_blas;
+}
+
+}  // namespace
+
+void build_kernel_matrix(const SyncPlan& plan, const float* d_gbeta, int ng, int dist_log,
+                         float* d_k, hipStream_t st) {
+  const dim3 grid(static_cast<unsigned>(ceil_div(plan.nbins, 256)), static_cast<unsigned>(ng));
+  kernel_matrix<<<grid, 256, 0, st>>>(device_view(plan), d_gbeta, ng, dist_log, d_k);
+  RGN_HIP(hipGetLastError());
+}
+
+}  // namespace rgn
+
+using namespace rgn;
+
+extern "C" int rgn_sync_fromdist_batched(const rgn_sync_plan* plan, const float* d_gbeta, int ng,
+                                         int dist_log, const float* d_f, size_t nd, int precision,
+                                         float* d_out, rgn_stream_t stream) {
+  return guarded([&] {
+    RGN_REQUIRE(plan != nullptr, RGN_ERR_INVALID, "null plan");
+    RGN_REQUIRE(d_gbeta && d_f && d_out, RGN_ERR_INVALID, "null argument");
+    RGN_REQUIRE(ng > 0, RGN_ERR_INVALID, "empty distribution grid");
+    const SyncPlan& p = plan->p;
+    hipStream_t st = as_stream(stream);
+    float* d_k = static_cast<float*>(scratch(sizeof(float) * static_cast<size_t>(ng) * p.nbins, 5));
+    build_kernel_matrix(p, d_gbeta, ng, dist_log, d_k, st);
+    if (nd == 0) return;
+    if (precision == RGN_PREC_FP32) {
+      std::lock_guard<std::mutex> lk(g_blas_mutex);
+      rocblas_handle h = blas();
+      RGN_REQUIRE(rocblas_set_stream(h, st) == rocblas_status_success, RGN_ERR_HIP,
+                  "rocblas_set_stream failed");
+      const float one = 1.0f, zero = 0.0f;
+      // row-major out[nd][nb] = f[nd][ng] K[ng][nb]  <=>  column-major out^T = K^T f^T
+      const rocblas_status s = rocblas_sgemm(h, rocblas_operation_none, rocblas_operation_none,
+                                             p.nbins, static_cast<rocblas_int>(nd), ng, &one, d_k,
+                                             p.nbins, d_f, ng, &zero, d_out, p.nbins);
+      RGN_REQUIRE(s == rocblas_status_success, RGN_ERR_HIP, "rocblas_sgemm failed");
+    } else if (precision == RGN_PREC_F16X3 || precision == RGN_PREC_F16X1) {
+      contraction_f16(p, d_k, ng, d_f, nd, precision == RGN_PREC_F16X3 ? 3 : 1, d_out, st);
+    } else {
+      throw Error(RGN_ERR_INVALID, "unknown precision mode");
+    }
+  });
+}

@@ -21,6 +21,7 @@
 namespace rgn {
 
 const std::vector<float>& ftable_y(const rgn_ftable* t);
+const std::vector<float>& ftable_x(const rgn_ftable* t);
 float ftable_xmin(const rgn_ftable* t);
 float ftable_xmax(const rgn_ftable* t);
 
@@ -91,9 +92,9 @@ __device__ __forceinline__ void eval_sources(const float4* __restrict__ src, int
 
 struct ParticleSourceFn {
   ParticlesDev D;
-  __device__ Source operator()(size_t p, const PlanDev& P) const {
-    return particle_source(D, p, P);
-  }
+  using Raw = RawParticle;
+  __device__ Raw load(size_t p) const { return load_particle(D, p); }
+  __device__ Source make(const Raw& r, const PlanDev& P) const { return particle_source(r, P); }
 };
 
 // synchrotron.hpp:72-95: eps = e_at * gb * gb / (g_syn * g_syn) in float;
@@ -102,10 +103,13 @@ struct DistSourceFn {
   const float* gbeta;
   const float* f;
   int dist_log;
-  __device__ Source operator()(size_t g, const PlanDev& P) const {
-    const float gb = gbeta[g];
-    const float eps = P.e_at * gb * gb / (P.g_syn * P.g_syn);
-    const float W = dist_log ? f[g] * gb : f[g];
+  struct Raw {
+    float gb, f;
+  };
+  __device__ Raw load(size_t g) const { return Raw{gbeta[g], f[g]}; }
+  __device__ Source make(const Raw& r, const PlanDev& P) const {
+    const float eps = P.e_at * r.gb * r.gb / (P.g_syn * P.g_syn);
+    const float W = dist_log ? r.f * r.gb : r.f;
     return make_source(W, eps, P);
   }
 };
@@ -131,10 +135,15 @@ __global__ void __launch_bounds__(kThreads) forward_kernel(SourceFn fn, size_t n
   }
 
   const size_t stride = static_cast<size_t>(gridDim.x) * kThreads;
-  for (size_t base = static_cast<size_t>(blockIdx.x) * kThreads; base < n; base += stride) {
+  size_t base = static_cast<size_t>(blockIdx.x) * kThreads;
+  typename SourceFn::Raw raw{};
+  if (base + threadIdx.x < n) raw = fn.load(base + threadIdx.x);
+  for (; base < n; base += stride) {
     const size_t p = base + threadIdx.x;
     Source s{0.0f, P.v_safe, 0.0f};
-    if (p < n) s = fn(p, P);
+    if (p < n) s = fn.make(raw, P);
+    // the next tile's inputs load while this tile is evaluated
+    if (p + stride < n) raw = fn.load(p + stride);
     __syncthreads();  // the previous tile has been consumed
     tile[threadIdx.x] = make_float4(s.W, s.vhi, s.vlo, 0.0f);
     __syncthreads();
@@ -194,7 +203,22 @@ struct GradOut {
   float* ge[3];
 };
 
-template <int NB>
+// Sum over groups of LPP lanes (16 or 32); the group total lands in the
+// group's last lane (for 16, in every lane of the group).
+template <int LPP>
+__device__ __forceinline__ float group_sum(float v) {
+  v += dpp_f<kDppQuadPerm1032>(v);
+  v += dpp_f<kDppQuadPerm2301>(v);
+  v += dpp_f<kDppRowHalfMirror>(v);
+  v += dpp_f<kDppRowMirror>(v);
+  if constexpr (LPP == 32) v += dpp_f<kDppRowBcast15, 0xa>(v);
+  return v;
+}
+
+// LPP lanes share one source; a wave evaluates 64 / LPP sources at a time, so
+// the per-source reduction (2 x log2(LPP) DPP adds) is amortised over the
+// group's LPP x NBL bins.  Lane i of a group owns bins e = i + LPP k, k < NBL.
+template <int LPP, int NBL>
 __global__ void __launch_bounds__(kThreads) adjoint_kernel(ParticlesDev D, size_t n, PlanDev P,
                                                            const float* __restrict__ gbar,
                                                            GradOut G) {
@@ -204,12 +228,15 @@ __global__ void __launch_bounds__(kThreads) adjoint_kernel(ParticlesDev D, size_
   float2* ltab = reinterpret_cast<float2*>(smem + kThreads * (sizeof(float4) + sizeof(float2)));
   for (int j = threadIdx.x; j < P.TP; j += kThreads) ltab[j] = P.table[j];
 
+  constexpr int PPW = 64 / LPP;  // sources per wave step
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  float c[NB], S[NB], g[NB];
+  const int grp = lane / LPP;
+  const int li = lane % LPP;
+  float c[NBL], S[NBL], g[NBL];
 #pragma unroll
-  for (int k = 0; k < NB; ++k) {
-    const int e = lane + 64 * k;
+  for (int k = 0; k < NBL; ++k) {
+    const int e = li + LPP * k;
     const float2 cs = P.bincs[e];
     c[k] = cs.x;
     S[k] = cs.y;
@@ -218,37 +245,41 @@ __global__ void __launch_bounds__(kThreads) adjoint_kernel(ParticlesDev D, size_
   const double dlnx = -P.K / 0.69314718055994530942;  // dF/dln x = s * du/dln x
 
   const size_t stride = static_cast<size_t>(gridDim.x) * kThreads;
-  for (size_t base = static_cast<size_t>(blockIdx.x) * kThreads; base < n; base += stride) {
+  size_t base = static_cast<size_t>(blockIdx.x) * kThreads;
+  RawParticle raw{};
+  if (base + threadIdx.x < n) raw = load_particle(D, base + threadIdx.x);
+  for (; base < n; base += stride) {
     const size_t p = base + threadIdx.x;
+    const RawParticle cur = raw;  // kept for the gradient epilogue
     Source s{0.0f, P.v_safe, 0.0f};
-    if (p < n) s = particle_source(D, p, P);
+    if (p < n) s = particle_source(cur, P);
+    if (p + stride < n) raw = load_particle(D, p + stride);  // next tile, in flight
     __syncthreads();
     tile[threadIdx.x] = make_float4(s.W, s.vhi, s.vlo, 0.0f);
     __syncthreads();
-    const float4* src = tile + wave * 64;
-    for (int j = 0; j < 64; ++j) {
-      const float4 q = src[j];
+    for (int step = 0; step < 64 / PPW; ++step) {
+      const int j = wave * 64 + step * PPW + grp;
+      const float4 q = tile[j];
       float a = 0.0f, cc = 0.0f;
 #pragma unroll
-      for (int k = 0; k < NB; ++k) {
+      for (int k = 0; k < NBL; ++k) {
         const float u = fmaf(c[k], S[k], q.y) + q.z;
         const float2 cs = ltab[cvt_sat_u32(u)];
         a = fmaf(g[k], fmaf(u, cs.y, cs.x), a);
         cc = fmaf(g[k], cs.y, cc);
       }
-      a = wave_sum_to_lane63(a);
-      cc = wave_sum_to_lane63(cc);
-      if (lane == 63) ac[wave * 64 + j] = make_float2(a, cc);
+      a = group_sum<LPP>(a);
+      cc = group_sum<LPP>(cc);
+      if (li == LPP - 1) ac[j] = make_float2(a, cc);
     }
     __syncthreads();
     if (p >= n) continue;
     const float2 mine = ac[threadIdx.x];
     const bool live = s.W != 0.0f;
-    const double ux = D.u[0][p], uy = D.u[1][p], uz = D.u[2][p];
-    const double ex = load_or_zero(D.e[0], p), ey = load_or_zero(D.e[1], p),
-                 ez = load_or_zero(D.e[2], p);
-    const double bx = D.b[0][p], by = D.b[1][p], bz = D.b[2][p];
-    const double w = D.w ? D.w[p] : 1.0;
+    const double ux = cur.ux, uy = cur.uy, uz = cur.uz;
+    const double ex = cur.ex, ey = cur.ey, ez = cur.ez;
+    const double bx = cur.bx, by = cur.by, bz = cur.bz;
+    const double w = cur.w;
     double gw = 0, gux = 0, guy = 0, guz = 0, gbx = 0, gby = 0, gbz = 0, gex = 0, gey = 0,
            gez = 0;
     if (live) {
@@ -314,21 +345,22 @@ __global__ void __launch_bounds__(kThreads) particle_terms_kernel(ParticlesDev D
   out[5 * p + 4] = s.vlo;
 }
 
-ParticlesDev make_particles(const float* const d_u[3], const float* const d_e[3],
-                            const float* const d_b[3], const float* d_w) {
+}  // namespace
+
+// Component pointers may be null only when there are no particles.
+ParticlesDev make_particles_checked(const float* const d_u[3], const float* const d_e[3],
+                                    const float* const d_b[3], const float* d_w, size_t n) {
   ParticlesDev D{};
   for (int k = 0; k < 3; ++k) {
-    RGN_REQUIRE(d_u && d_u[k], RGN_ERR_INVALID, "null momentum component");
-    RGN_REQUIRE(d_b && d_b[k], RGN_ERR_INVALID, "null magnetic-field component");
-    D.u[k] = d_u[k];
-    D.b[k] = d_b[k];
+    RGN_REQUIRE(n == 0 || (d_u && d_u[k]), RGN_ERR_INVALID, "null momentum component");
+    RGN_REQUIRE(n == 0 || (d_b && d_b[k]), RGN_ERR_INVALID, "null magnetic-field component");
+    D.u[k] = d_u ? d_u[k] : nullptr;
+    D.b[k] = d_b ? d_b[k] : nullptr;
     D.e[k] = d_e ? d_e[k] : nullptr;
   }
   D.w = d_w;
   return D;
 }
-
-}  // namespace
 
 // ---------------------------------------------------------------------------
 // Plan
@@ -379,7 +411,12 @@ void build_sync_plan(SyncPlan& plan, const rgn_ftable* table, const float* h_bin
               "photon bins span too many decades for the LDS table");
   plan.TP = plan.Z + static_cast<int>(std::ceil(span)) + 4;
 
-  // Padded segment table in u = Z - t.
+  // Padded segment table in u = Z - t.  Segment j interpolates between nodes
+  // a = Z - j - 1 and b = Z - j at their true coordinates: the float Logspace
+  // nodes sit up to a few 1e-6 table units off the uniform grid, and the
+  // reference interpolates between the float nodes (tabulation.hpp:39-41).
+  const std::vector<float>& xs = ftable_x(table);
+  auto node_u = [&](int i) { return plan.Z - (std::log2(static_cast<double>(xs[i])) - plan.log2_xmin) * K; };
   std::vector<float2> tab(plan.TP);
   for (int j = 0; j < plan.TP; ++j) {
     const int a = plan.Z - j - 1, b = plan.Z - j;
@@ -387,7 +424,9 @@ void build_sync_plan(SyncPlan& plan, const rgn_ftable* table, const float* h_bin
       tab[j] = make_float2(0.0f, 0.0f);
     } else {
       const double ya = y[a], yb = y[b];
-      tab[j] = make_float2(static_cast<float>(yb + j * (yb - ya)), static_cast<float>(ya - yb));
+      const double ua = node_u(a), ub = node_u(b);
+      const double s = (ya - yb) / (ua - ub);
+      tab[j] = make_float2(static_cast<float>(yb - ub * s), static_cast<float>(s));
     }
   }
   plan.table.reset(plan.TP * sizeof(float2));
@@ -396,21 +435,24 @@ void build_sync_plan(SyncPlan& plan, const rgn_ftable* table, const float* h_bin
   // Per-bin coordinates; dead bins and padding lanes reuse the lowest live A.
   plan.NB = static_cast<int>(ceil_div(nbins, 64));
   plan.nbins_pad = 64 * plan.NB;
-  std::vector<double> U(plan.nbins_pad, plan.Z - amin);
-  std::vector<float> bine(plan.nbins_pad, 0.0f);
+  // the per-bin arrays also cover the adjoint's lane layouts (64, 128, 256 or
+  // 512 bins, see rgn_sync_adjoint)
+  const int adj_cover = nbins <= 64 ? 64 : nbins <= 128 ? 128 : nbins <= 256 ? 256 : 512;
+  const int alloc = std::max(plan.nbins_pad, adj_cover);
+  std::vector<double> U(alloc, plan.Z - amin);
+  std::vector<float> bine(alloc, 0.0f);
   for (int e = 0; e < nbins; ++e) {
     if (live[e]) {
       U[e] = plan.Z - A[e];
       bine[e] = h_bins[e];
     }
   }
-  DeviceBuffer dU(plan.nbins_pad * sizeof(double));
+  DeviceBuffer dU(alloc * sizeof(double));
   RGN_HIP(hipMemcpy(dU.ptr, U.data(), dU.bytes, hipMemcpyHostToDevice));
-  plan.bincs.reset(plan.nbins_pad * sizeof(float2));
-  product_pair_kernel<<<plan.nbins_pad, 64>>>(dU.as<double>(), plan.bincs.as<float2>(),
-                                              plan.nbins_pad);
+  plan.bincs.reset(alloc * sizeof(float2));
+  product_pair_kernel<<<alloc, 64>>>(dU.as<double>(), plan.bincs.as<float2>(), alloc);
   RGN_HIP(hipGetLastError());
-  plan.bine.reset(plan.nbins_pad * sizeof(float));
+  plan.bine.reset(alloc * sizeof(float));
   RGN_HIP(hipMemcpy(plan.bine.ptr, bine.data(), plan.bine.bytes, hipMemcpyHostToDevice));
   RGN_HIP(hipDeviceSynchronize());  // dU goes out of scope
 
@@ -496,10 +538,6 @@ void sync_forward(const SyncPlan& plan, const ParticlesDev& D, size_t n, float* 
 
 using namespace rgn;
 
-struct rgn_sync_plan {
-  SyncPlan p;
-};
-
 extern "C" {
 
 int rgn_sync_plan_create(const rgn_ftable* table, const float* h_bins, int nbins, float B0,
@@ -519,6 +557,7 @@ int rgn_sync_plan_create(const rgn_ftable* table, const float* h_bins, int nbins
 }
 
 int rgn_sync_plan_destroy(rgn_sync_plan* plan) {
+  if (plan) forget_plan_graphs(&plan->p);
   delete plan;
   return RGN_OK;
 }
@@ -528,8 +567,9 @@ int rgn_sync_spectrum(const rgn_sync_plan* plan, const float* const d_u[3],
                       size_t n, float* d_out, rgn_stream_t stream) {
   return guarded([&] {
     RGN_REQUIRE(plan != nullptr, RGN_ERR_INVALID, "null plan");
-    const ParticlesDev D = make_particles(d_u, d_e, d_b, d_w);
-    sync_forward(plan->p, D, n, d_out, as_stream(stream), 4);
+    const ParticlesDev D = make_particles_checked(d_u, d_e, d_b, d_w, n);
+    // 16 blocks per CU measured fastest on MI355X (2^24 x 256: 0.68 ms vs 0.74 at 4)
+    sync_forward(plan->p, D, n, d_out, as_stream(stream), env_int("RAGNAR_FWD_BPC", 16));
   });
 }
 
@@ -540,7 +580,7 @@ int rgn_sync_adjoint(const rgn_sync_plan* plan, const float* const d_u[3],
   return guarded([&] {
     RGN_REQUIRE(plan != nullptr, RGN_ERR_INVALID, "null plan");
     RGN_REQUIRE(d_gbar != nullptr, RGN_ERR_INVALID, "null upstream gradient");
-    const ParticlesDev D = make_particles(d_u, d_e, d_b, d_w);
+    const ParticlesDev D = make_particles_checked(d_u, d_e, d_b, d_w, n);
     GradOut G{};
     G.gw = d_gw;
     for (int k = 0; k < 3; ++k) {
@@ -551,25 +591,19 @@ int rgn_sync_adjoint(const rgn_sync_plan* plan, const float* const d_u[3],
     if (n == 0) return;
     const SyncPlan& p = plan->p;
     const PlanDev P = device_view(p);
-    const int grid = forward_grid(n, 4);
+    const int grid = forward_grid(n, env_int("RAGNAR_ADJ_BPC", 16));
     const size_t smem = kThreads * (sizeof(float4) + sizeof(float2)) + p.TP * sizeof(float2);
     hipStream_t st = as_stream(stream);
-    switch (p.NB) {
-#define RGN_ADJ_CASE(NB_)                                                              \
-  case NB_:                                                                            \
-    adjoint_kernel<NB_><<<grid, kThreads, smem, st>>>(D, n, P, d_gbar, G);             \
-    break;
-      RGN_ADJ_CASE(1)
-      RGN_ADJ_CASE(2)
-      RGN_ADJ_CASE(3)
-      RGN_ADJ_CASE(4)
-      RGN_ADJ_CASE(5)
-      RGN_ADJ_CASE(6)
-      RGN_ADJ_CASE(7)
-      RGN_ADJ_CASE(8)
-#undef RGN_ADJ_CASE
-      default:
-        throw Error(RGN_ERR_UNSUPPORTED, "at most 512 photon bins per plan");
+    // (lanes per source, bins per lane) covering nbins; the plan pads its
+    // per-bin arrays to cover every configuration (adjoint_coverage).
+    if (p.nbins <= 64) {
+      adjoint_kernel<16, 4><<<grid, kThreads, smem, st>>>(D, n, P, d_gbar, G);
+    } else if (p.nbins <= 128) {
+      adjoint_kernel<16, 8><<<grid, kThreads, smem, st>>>(D, n, P, d_gbar, G);
+    } else if (p.nbins <= 256) {
+      adjoint_kernel<16, 16><<<grid, kThreads, smem, st>>>(D, n, P, d_gbar, G);
+    } else {
+      adjoint_kernel<32, 16><<<grid, kThreads, smem, st>>>(D, n, P, d_gbar, G);
     }
     RGN_HIP(hipGetLastError());
   });
@@ -599,7 +633,7 @@ int rgn_sync_particle_terms(const rgn_sync_plan* plan, const float* const d_u[3]
                             const float* d_w, size_t n, float* d_out, rgn_stream_t stream) {
   return guarded([&] {
     RGN_REQUIRE(plan != nullptr, RGN_ERR_INVALID, "null plan");
-    const ParticlesDev D = make_particles(d_u, d_e, d_b, d_w);
+    const ParticlesDev D = make_particles_checked(d_u, d_e, d_b, d_w, n);
     if (!n) return;
     particle_terms_kernel<<<static_cast<unsigned>(ceil_div(n, kThreads)), kThreads, 0,
                             as_stream(stream)>>>(D, n, device_view(plan->p), d_out);

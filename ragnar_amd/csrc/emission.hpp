@@ -32,4 +32,15 @@ int forward_grid(size_t n, int blocks_per_cu);
 void launch_combine(const SyncPlan& plan, const double* partials, int grid, float* out,
                     hipStream_t st);
 
+// Particle pointers of the C ABI; they may be null only when n == 0.
+ParticlesDev make_particles_checked(const float* const d_u[3], const float* const d_e[3],
+                                    const float* const d_b[3], const float* d_w, size_t n);
+
+// Drops the cached cell graphs that reference a plan being destroyed.
+void forget_plan_graphs(const void* plan);
+
 }  // namespace rgn
+
+struct rgn_sync_plan {
+  rgn::SyncPlan p;
+};

@@ -6,6 +6,8 @@
 //   t(x) = (log2 x - log2 x_min) * K,   K = (n - 1) / log2(x_max / x_min),
 // node i sits at t = i.  For a photon bin e and a particle p, x = e_e / eps_p,
 // so t = A_e - B_p with A_e = (log2 e_e - log2 x_min) K and B_p = log2(eps_p) K.
+// (The float nodes sit within a few 1e-6 of integer t; each segment of the
+// device table interpolates between the two nodes' exact coordinates.)
 // The kernels use the reversed coordinate u = Z - t = U_e + B_p, where Z is the
 // first node of the table's all-zero tail (183 for the reference table).  In
 // u the F table is a padded array of segments: segment j covers u in [j, j+1)
@@ -107,17 +109,37 @@ __device__ __forceinline__ float load_or_zero(const float* p, size_t i) {
   return p ? p[i] : 0.0f;
 }
 
+// The raw inputs of one particle (loaded ahead of their use).
+struct RawParticle {
+  float ux, uy, uz, ex, ey, ez, bx, by, bz, w;
+};
+
+__device__ __forceinline__ RawParticle load_particle(const ParticlesDev& D, size_t p) {
+  RawParticle r;
+  r.ux = D.u[0][p];
+  r.uy = D.u[1][p];
+  r.uz = D.u[2][p];
+  r.ex = load_or_zero(D.e[0], p);
+  r.ey = load_or_zero(D.e[1], p);
+  r.ez = load_or_zero(D.e[2], p);
+  r.bx = D.b[0][p];
+  r.by = D.b[1][p];
+  r.bz = D.b[2][p];
+  r.w = D.w ? D.w[p] : 1.0f;
+  return r;
+}
+
+__device__ __forceinline__ Source particle_source(const RawParticle& r, const PlanDev& P) {
+  float e_peak, chiR;
+  omega_sync(r.ux, r.uy, r.uz, r.ex, r.ey, r.ez, r.bx, r.by, r.bz, P.B0, P.g_syn, P.e_at, e_peak,
+             chiR);
+  return make_source(chiR * r.w, e_peak, P);
+}
+
 // Source of particle p of a particle store.
 __device__ __forceinline__ Source particle_source(const ParticlesDev& D, size_t p,
                                                   const PlanDev& P) {
-  const float ux = D.u[0][p], uy = D.u[1][p], uz = D.u[2][p];
-  const float ex = load_or_zero(D.e[0], p), ey = load_or_zero(D.e[1], p),
-              ez = load_or_zero(D.e[2], p);
-  const float bx = D.b[0][p], by = D.b[1][p], bz = D.b[2][p];
-  float e_peak, chiR;
-  omega_sync(ux, uy, uz, ex, ey, ez, bx, by, bz, P.B0, P.g_syn, P.e_at, e_peak, chiR);
-  const float w = D.w ? D.w[p] : 1.0f;
-  return make_source(chiR * w, e_peak, P);
+  return particle_source(load_particle(D, p), P);
 }
 
 // DPP row/bank controls (GFX9 encoding).

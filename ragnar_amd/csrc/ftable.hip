@@ -33,6 +33,7 @@ namespace rgn {
 
 // Shared with the emission plans (declared in emission.hip).
 const std::vector<float>& ftable_y(const rgn_ftable* t) { return t->y; }
+const std::vector<float>& ftable_x(const rgn_ftable* t) { return t->x; }
 float ftable_xmin(const rgn_ftable* t) { return t->xmin; }
 float ftable_xmax(const rgn_ftable* t) { return t->xmax; }
 

@@ -83,47 +83,95 @@ void build_edges(rgn_hist_plan& plan) {
   }
 }
 
+struct HistParams {
+  const double* edges;  // in LDS
+  int nb;
+  float emin, emax, scale;
+  int log_bins, fourvel;
+};
+
+// particles.cpp:228-252 for one particle: bin index and the float term.
+__device__ __forceinline__ void bin_particle(float x, float y, float z, const HistParams& H,
+                                             double* __restrict__ my_sum,
+                                             unsigned* __restrict__ my_cnt) {
+  const float usqr = x * x + y * y + z * z;  // float, as the reference
+  const double energy = H.fourvel ? static_cast<double>(sqrtf(usqr))
+                                  : sqrt(1.0 + static_cast<double>(usqr));
+  int idx;
+  if (energy < H.emin) {
+    idx = 0;
+  } else if (energy >= H.emax) {
+    idx = H.nb - 1;
+  } else if (energy == energy) {
+    const float est = H.scale * log2f(static_cast<float>(energy) / H.emin);
+    idx = static_cast<int>(fminf(fmaxf(est, 0.0f), static_cast<float>(H.nb - 1)));
+    while (idx < H.nb - 1 && energy >= H.edges[idx + 1]) ++idx;
+    while (idx > 0 && energy < H.edges[idx]) --idx;
+  } else {
+    idx = H.nb - 1;  // NaN energy: the x86 reference's size_t conversion lands here
+  }
+  const float term = H.log_bins ? static_cast<float>(1.0 / energy) : 1.0f;
+  atomicAdd(my_cnt + idx, 1u);
+  atomicAdd(my_sum + idx, static_cast<double>(term));
+}
+
+// Each thread keeps two 16-byte loads per component in flight (8 particles)
+// when the three component arrays are 16-byte aligned; the remainder (and
+// unaligned inputs) go through the scalar loop.
 __global__ void __launch_bounds__(kThreads) hist_kernel(const float* __restrict__ ux,
                                                         const float* __restrict__ uy,
                                                         const float* __restrict__ uz, size_t n,
-                                                        const double* __restrict__ edges, int nb,
+                                                        const double* __restrict__ g_edges, int nb,
                                                         float emin, float emax, float scale,
-                                                        int log_bins, int fourvel,
+                                                        int log_bins, int fourvel, int vec,
                                                         unsigned long long* __restrict__ counts,
                                                         double* __restrict__ partials) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int wave = threadIdx.x >> 6;
-  double* lsum = reinterpret_cast<double*>(smem);                      // [4][nb]
-  unsigned* lcnt = reinterpret_cast<unsigned*>(smem + 4 * nb * sizeof(double));  // [4][nb]
+  double* lsum = reinterpret_cast<double*>(smem);                                 // [4][nb]
+  double* ledges = lsum + 4 * nb;                                                 // [nb + 1]
+  unsigned* lcnt = reinterpret_cast<unsigned*>(ledges + nb + 1);                  // [4][nb]
   for (int i = threadIdx.x; i < 4 * nb; i += kThreads) {
     lsum[i] = 0.0;
     lcnt[i] = 0u;
   }
+  for (int i = threadIdx.x; i <= nb; i += kThreads) ledges[i] = g_edges[i];
   __syncthreads();
+  const HistParams H{ledges, nb, emin, emax, scale, log_bins, fourvel};
   double* my_sum = lsum + wave * nb;
   unsigned* my_cnt = lcnt + wave * nb;
   const size_t stride = static_cast<size_t>(gridDim.x) * kThreads;
-  for (size_t p = blockIdx.x * static_cast<size_t>(kThreads) + threadIdx.x; p < n; p += stride) {
-    const float x = ux[p], y = uy[p], z = uz[p];
-    const float usqr = x * x + y * y + z * z;  // particles.cpp:228-230, float
-    const double energy = fourvel ? static_cast<double>(sqrtf(usqr))
-                                  : sqrt(1.0 + static_cast<double>(usqr));
-    int idx;
-    if (energy < emin) {
-      idx = 0;
-    } else if (energy >= emax) {
-      idx = nb - 1;
-    } else if (energy == energy) {
-      const float est = scale * log2f(static_cast<float>(energy) / emin);
-      idx = static_cast<int>(fminf(fmaxf(est, 0.0f), static_cast<float>(nb - 1)));
-      while (idx < nb - 1 && energy >= edges[idx + 1]) ++idx;
-      while (idx > 0 && energy < edges[idx]) --idx;
-    } else {
-      idx = nb - 1;  // NaN energy: the x86 reference's size_t conversion lands here
+  const size_t tid = blockIdx.x * static_cast<size_t>(kThreads) + threadIdx.x;
+  size_t scalar_from = 0;
+  if (vec) {
+    const size_t nq = n / 4;
+    const float4* x4 = reinterpret_cast<const float4*>(ux);
+    const float4* y4 = reinterpret_cast<const float4*>(uy);
+    const float4* z4 = reinterpret_cast<const float4*>(uz);
+    for (size_t q = tid; q < nq; q += 2 * stride) {
+      const bool two = q + stride < nq;
+      const float4 a0 = x4[q], b0 = y4[q], c0 = z4[q];
+      float4 a1 = a0, b1 = b0, c1 = c0;
+      if (two) {
+        a1 = x4[q + stride];
+        b1 = y4[q + stride];
+        c1 = z4[q + stride];
+      }
+      bin_particle(a0.x, b0.x, c0.x, H, my_sum, my_cnt);
+      bin_particle(a0.y, b0.y, c0.y, H, my_sum, my_cnt);
+      bin_particle(a0.z, b0.z, c0.z, H, my_sum, my_cnt);
+      bin_particle(a0.w, b0.w, c0.w, H, my_sum, my_cnt);
+      if (two) {
+        bin_particle(a1.x, b1.x, c1.x, H, my_sum, my_cnt);
+        bin_particle(a1.y, b1.y, c1.y, H, my_sum, my_cnt);
+        bin_particle(a1.z, b1.z, c1.z, H, my_sum, my_cnt);
+        bin_particle(a1.w, b1.w, c1.w, H, my_sum, my_cnt);
+      }
     }
-    const float term = log_bins ? static_cast<float>(1.0 / energy) : 1.0f;
-    atomicAdd(my_cnt + idx, 1u);
-    atomicAdd(my_sum + idx, static_cast<double>(term));
+    scalar_from = 4 * nq;
+  }
+  for (size_t p = scalar_from + tid; p < n; p += stride) {
+    bin_particle(ux[p], uy[p], uz[p], H, my_sum, my_cnt);
   }
   __syncthreads();
   for (int i = threadIdx.x; i < nb; i += kThreads) {
@@ -209,14 +257,17 @@ int rgn_energy_hist(const rgn_hist_plan* plan, const float* const d_u[3], size_t
     RGN_REQUIRE(d_u && d_u[0] && d_u[1] && d_u[2] && d_hist, RGN_ERR_INVALID, "null argument");
     hipStream_t st = as_stream(stream);
     const int nb = plan->nb;
-    const size_t tiles = ceil_div(std::max<size_t>(n, 1), kThreads);
-    const int grid = static_cast<int>(std::min<size_t>(tiles, static_cast<size_t>(num_cus()) * 4));
+    auto aligned16 = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; };
+    const int vec = aligned16(d_u[0]) && aligned16(d_u[1]) && aligned16(d_u[2]);
+    const size_t per_thread = vec ? 8 : 1;
+    const size_t tiles = ceil_div(std::max<size_t>(n, 1), kThreads * per_thread);
+    const int grid = static_cast<int>(std::min<size_t>(tiles, static_cast<size_t>(num_cus()) * 8));
     double* partials = static_cast<double*>(scratch(sizeof(double) * nb * static_cast<size_t>(grid), 2));
     if (d_counts) RGN_HIP(hipMemsetAsync(d_counts, 0, sizeof(uint64_t) * nb, st));
-    const size_t smem = 4 * nb * (sizeof(double) + sizeof(unsigned));
+    const size_t smem = 4 * nb * (sizeof(double) + sizeof(unsigned)) + (nb + 1) * sizeof(double);
     hist_kernel<<<grid, kThreads, smem, st>>>(
       d_u[0], d_u[1], d_u[2], n, plan->d_edges.as<double>(), nb, plan->emin, plan->emax,
-      plan->approx_scale, plan->log_bins, fourvel ? 1 : 0,
+      plan->approx_scale, plan->log_bins, fourvel ? 1 : 0, vec,
       reinterpret_cast<unsigned long long*>(d_counts), partials);
     RGN_HIP(hipGetLastError());
     hist_combine_kernel<<<nb, kThreads, 0, st>>>(partials, grid, nb, d_hist);

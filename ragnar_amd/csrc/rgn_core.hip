@@ -2,6 +2,7 @@
 // the per-device scratch cache.
 #include "rgn_core.hpp"
 
+#include <cstdlib>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -55,6 +56,13 @@ void ensure_init() {
 }
 
 int num_cus() { return g_num_cus > 0 ? g_num_cus : 256; }
+
+int env_int(const char* name, int def) {
+  const char* v = std::getenv(name);
+  if (!v || !*v) return def;
+  const int x = std::atoi(v);
+  return x > 0 ? x : def;
+}
 
 void DeviceBuffer::reset(size_t n) {
   release();

@@ -78,6 +78,9 @@ void* scratch(size_t bytes, int slot);
 // Number of compute units of the current device.
 int num_cus();
 
+// Integer from the environment (launch-shape knobs), or def when unset.
+int env_int(const char* name, int def);
+
 inline hipStream_t as_stream(rgn_stream_t s) { return static_cast<hipStream_t>(s); }
 
 inline size_t ceil_div(size_t a, size_t b) { return (a + b - 1) / b; }

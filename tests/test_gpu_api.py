@@ -4,6 +4,8 @@ module ``ragnar_amd.ragnar`` (same calls and assertions, GPU backend).
 import numpy as np
 import pytest
 
+from _spectra import assert_spectrum, oracle_pair
+
 pytestmark = pytest.mark.gpu
 
 
@@ -211,9 +213,8 @@ def test_sync_prtls(rg, orc):
     b = np.stack([b1s, b2s, b3s]).astype(np.float32)
     _, hd, _ = orc.energy_hist(u, orc.logspace(1, 1e3, 100), True)
     assert np.allclose(dist_prtls.F().as_array(), hd, rtol=1e-6, atol=0)
-    _, refd = orc.sync_spectrum(u, b, orc.logspace(0.01, 1e5, 200), want_fp64=True)
-    live = refd > 1e-12 * refd.max()
-    assert np.max(np.abs(s_p.as_array()[live] / refd[live] - 1)) <= 1e-5
+    ex, f64 = oracle_pair(orc, u, b, orc.logspace(0.01, 1e5, 200))
+    assert_spectrum(s_p.as_array(), ex, f64, what="sync_prtls")
 
 
 def test_sync_unit_check_and_dims(rg):
@@ -235,10 +236,8 @@ def test_more_than_512_bins(rg, orc):
     prtls.fromArrays({"U1": u[0], "U2": u[1], "U3": u[2], "B1": b[0], "B2": b[1], "B3": b[2]})
     bins = rg.Logbins(1e-2, 1e7, 1200, "mec2")
     s = rg.SynchrotronSpectrum_3D(prtls, bins, 1, 1, 1).as_array()
-    _, refd = orc.sync_spectrum(u.astype(np.float32), b.astype(np.float32), bins.as_array(),
-                                want_fp64=True)
-    live = refd > 1e-12 * refd.max()
-    assert np.max(np.abs(s[live] / refd[live] - 1)) <= 1e-5
+    ex, f64 = oracle_pair(orc, u.astype(np.float32), b.astype(np.float32), bins.as_array())
+    assert_spectrum(s, ex, f64, what="1200 bins")
 
 
 def test_weights_extension(rg, orc):
@@ -251,7 +250,6 @@ def test_weights_extension(rg, orc):
     assert p.has_weights()
     bins = rg.Logbins(1e-2, 1e7, 128, "mec2")
     s = rg.SynchrotronSpectrum_3D(p, bins, 1, 1, 1).as_array()
-    _, refd = orc.sync_spectrum(u.astype(np.float32), b.astype(np.float32), bins.as_array(),
-                                w=w.astype(np.float32), want_fp64=True)
-    live = refd > 1e-12 * refd.max()
-    assert np.max(np.abs(s[live] / refd[live] - 1)) <= 1e-5
+    ex, f64 = oracle_pair(orc, u.astype(np.float32), b.astype(np.float32), bins.as_array(),
+                          w=w.astype(np.float32))
+    assert_spectrum(s, ex, f64, what="weights")
