@@ -109,6 +109,25 @@ int rgn_sync_adjoint(const rgn_sync_plan* plan, const float* const d_u[3],
                      size_t n, const float* d_gbar, float* d_gw, float* const d_gu[3],
                      float* const d_gb[3], float* const d_ge[3], rgn_stream_t stream);
 
+/* Method selection for the two calls above.  RGN_METHOD_DIRECT evaluates every
+ * particle x bin pair (VALU bound); RGN_METHOD_BINNED uses the exact
+ * factorisation of the piecewise-linear interpolant: particles are partitioned
+ * by the integer part of their table coordinate and reduced into per-sub-cell
+ * moments (forward), or read one precomputed entry each (adjoint); cost is
+ * ~independent of the bin count.  RGN_METHOD_AUTO picks binned from 2^28 pairs
+ * (env RAGNAR_SYNC_METHOD=direct|binned overrides).  Same results to
+ * FP32/FP64 rounding (no reference counterpart). */
+enum { RGN_METHOD_AUTO = 0, RGN_METHOD_DIRECT = 1, RGN_METHOD_BINNED = 2 };
+int rgn_sync_spectrum_method(const rgn_sync_plan* plan, const float* const d_u[3],
+                             const float* const d_e[3], const float* const d_b[3],
+                             const float* d_w, size_t n, int method, float* d_out,
+                             rgn_stream_t stream);
+int rgn_sync_adjoint_method(const rgn_sync_plan* plan, const float* const d_u[3],
+                            const float* const d_e[3], const float* const d_b[3],
+                            const float* d_w, size_t n, const float* d_gbar, int method,
+                            float* d_gw, float* const d_gu[3], float* const d_gb[3],
+                            float* const d_ge[3], rgn_stream_t stream);
+
 /* Spatially resolved spectra: cell c owns particles [d_offsets[c], d_offsets[c+1]);
  * d_out[c * nbins + e].  One persistent launch plus a fixed-order combine.
  * h_offsets: host copy of the ncells+1 offsets (used to balance the launch).

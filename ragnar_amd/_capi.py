@@ -16,6 +16,7 @@ HEADER_PATH = _HERE.parent / "include" / "ragnar_hip.h"
 RGN_OK, RGN_ERR_INVALID, RGN_ERR_RANGE, RGN_ERR_HIP, RGN_ERR_NOMEM, RGN_ERR_UNSUPPORTED = range(6)
 RGN_FTABLE_REFERENCE, RGN_FTABLE_ACCURATE = 0, 1
 RGN_PREC_FP32, RGN_PREC_F16X3, RGN_PREC_F16X1 = 0, 1, 2
+RGN_METHOD_AUTO, RGN_METHOD_DIRECT, RGN_METHOD_BINNED = 0, 1, 2
 
 vp = C.c_void_p
 sz = C.c_size_t
@@ -42,6 +43,9 @@ _SIGNATURES = {
     "rgn_sync_spectrum": (C.c_int, [vp, fptr3, fptr3, fptr3, vp, sz, vp, vp]),
     "rgn_sync_adjoint": (C.c_int, [vp, fptr3, fptr3, fptr3, vp, sz, vp, vp, fptr3, fptr3,
                                    fptr3, vp]),
+    "rgn_sync_spectrum_method": (C.c_int, [vp, fptr3, fptr3, fptr3, vp, sz, C.c_int, vp, vp]),
+    "rgn_sync_adjoint_method": (C.c_int, [vp, fptr3, fptr3, fptr3, vp, sz, vp, C.c_int, vp, fptr3,
+                                          fptr3, fptr3, vp]),
     "rgn_sync_cells": (C.c_int, [vp, fptr3, fptr3, fptr3, vp, vp, C.c_int, vp, vp]),
     "rgn_sync_particle_terms": (C.c_int, [vp, fptr3, fptr3, fptr3, vp, sz, vp, vp]),
     "rgn_sync_plan_info": (C.c_int, [vp, vp, vp]),

@@ -25,8 +25,8 @@ CSRC = HERE / "csrc"
 BUILD = HERE / "_build"
 INCLUDE = ROOT / "include"
 
-HIP_SOURCES = ["rgn_core.hip", "ftable.hip", "emission.hip", "cells.hip", "histogram.hip",
-               "contraction.hip"]
+HIP_SOURCES = ["rgn_core.hip", "ftable.hip", "emission.hip", "binned.hip", "cells.hip",
+               "histogram.hip", "contraction.hip"]
 HOST_SOURCES = ["host/pyragnar.cpp", "host/containers.cpp", "host/h5io.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

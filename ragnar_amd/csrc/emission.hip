@@ -12,7 +12,9 @@
 // (c_j, s_j) segments; one eval is 6 VALU ops + one ds_read_b64.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <limits>
+#include <string>
 #include <vector>
 
 #include "emission.hpp"
@@ -459,6 +461,11 @@ void build_sync_plan(SyncPlan& plan, const rgn_ftable* table, const float* h_bin
   plan.win_lo = amin - plan.Z;
   plan.win_hi = amax;
   plan.v_safe = static_cast<float>(std::rint(amax));
+
+  plan.T = T;
+  plan.A = A;
+  plan.live = live;
+  build_binned_tables(plan, xs, y);
 }
 
 PlanDev device_view(const SyncPlan& plan) {
@@ -524,6 +531,22 @@ void launch_combine(const SyncPlan& plan, const double* partials, int grid, floa
   RGN_HIP(hipGetLastError());
 }
 
+// Direct pair-by-pair kernels below ~2^28 evals (the factorised form has a
+// fixed cost of seven launches); the factorised form above.  RAGNAR_SYNC_METHOD
+// (direct | binned) overrides the automatic choice.
+int resolve_method(const SyncPlan& plan, size_t n, int method) {
+  if (method == RGN_METHOD_AUTO) {
+    if (const char* m = std::getenv("RAGNAR_SYNC_METHOD")) {
+      if (std::string(m) == "direct") return RGN_METHOD_DIRECT;
+      if (std::string(m) == "binned") return RGN_METHOD_BINNED;
+    }
+    return static_cast<double>(n) * plan.nbins >= 268435456.0 ? RGN_METHOD_BINNED : RGN_METHOD_DIRECT;
+  }
+  RGN_REQUIRE(method == RGN_METHOD_DIRECT || method == RGN_METHOD_BINNED, RGN_ERR_INVALID,
+              "unknown method");
+  return method;
+}
+
 void sync_forward(const SyncPlan& plan, const ParticlesDev& D, size_t n, float* out,
                   hipStream_t st, int blocks_per_cu) {
   RGN_REQUIRE(out != nullptr, RGN_ERR_INVALID, "null output");
@@ -562,25 +585,50 @@ int rgn_sync_plan_destroy(rgn_sync_plan* plan) {
   return RGN_OK;
 }
 
+int rgn_sync_spectrum_method(const rgn_sync_plan* plan, const float* const d_u[3],
+                             const float* const d_e[3], const float* const d_b[3],
+                             const float* d_w, size_t n, int method, float* d_out,
+                             rgn_stream_t stream) {
+  return guarded([&] {
+    RGN_REQUIRE(plan != nullptr, RGN_ERR_INVALID, "null plan");
+    RGN_REQUIRE(d_out != nullptr, RGN_ERR_INVALID, "null output");
+    const ParticlesDev D = make_particles_checked(d_u, d_e, d_b, d_w, n);
+    if (resolve_method(plan->p, n, method) == RGN_METHOD_BINNED) {
+      sync_forward_binned(plan->p, D, n, d_out, as_stream(stream));
+    } else {
+      // 16 blocks per CU measured fastest on MI355X (2^24 x 256: 0.68 ms vs 0.74 at 4)
+      sync_forward(plan->p, D, n, d_out, as_stream(stream), env_int("RAGNAR_FWD_BPC", 16));
+    }
+  });
+}
+
 int rgn_sync_spectrum(const rgn_sync_plan* plan, const float* const d_u[3],
                       const float* const d_e[3], const float* const d_b[3], const float* d_w,
                       size_t n, float* d_out, rgn_stream_t stream) {
-  return guarded([&] {
-    RGN_REQUIRE(plan != nullptr, RGN_ERR_INVALID, "null plan");
-    const ParticlesDev D = make_particles_checked(d_u, d_e, d_b, d_w, n);
-    // 16 blocks per CU measured fastest on MI355X (2^24 x 256: 0.68 ms vs 0.74 at 4)
-    sync_forward(plan->p, D, n, d_out, as_stream(stream), env_int("RAGNAR_FWD_BPC", 16));
-  });
+  return rgn_sync_spectrum_method(plan, d_u, d_e, d_b, d_w, n, RGN_METHOD_AUTO, d_out, stream);
 }
 
 int rgn_sync_adjoint(const rgn_sync_plan* plan, const float* const d_u[3],
                      const float* const d_e[3], const float* const d_b[3], const float* d_w,
                      size_t n, const float* d_gbar, float* d_gw, float* const d_gu[3],
                      float* const d_gb[3], float* const d_ge[3], rgn_stream_t stream) {
+  return rgn_sync_adjoint_method(plan, d_u, d_e, d_b, d_w, n, d_gbar, RGN_METHOD_AUTO, d_gw,
+                                 d_gu, d_gb, d_ge, stream);
+}
+
+int rgn_sync_adjoint_method(const rgn_sync_plan* plan, const float* const d_u[3],
+                            const float* const d_e[3], const float* const d_b[3],
+                            const float* d_w, size_t n, const float* d_gbar, int method,
+                            float* d_gw, float* const d_gu[3], float* const d_gb[3],
+                            float* const d_ge[3], rgn_stream_t stream) {
   return guarded([&] {
     RGN_REQUIRE(plan != nullptr, RGN_ERR_INVALID, "null plan");
     RGN_REQUIRE(d_gbar != nullptr, RGN_ERR_INVALID, "null upstream gradient");
     const ParticlesDev D = make_particles_checked(d_u, d_e, d_b, d_w, n);
+    if (resolve_method(plan->p, n, method) == RGN_METHOD_BINNED) {
+      sync_adjoint_binned(plan->p, D, n, d_gbar, d_gw, d_gu, d_gb, d_ge, as_stream(stream));
+      return;
+    }
     GradOut G{};
     G.gw = d_gw;
     for (int k = 0; k < 3; ++k) {

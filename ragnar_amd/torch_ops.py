@@ -59,16 +59,22 @@ def _vec(t: torch.Tensor | None, name: str, n: int):
     return t.data_ptr()
 
 
-def sync_spectrum(plan: SyncPlan, u, b, w=None, e=None, out=None) -> torch.Tensor:
+METHODS = {"auto": _capi.RGN_METHOD_AUTO, "direct": _capi.RGN_METHOD_DIRECT,
+           "binned": _capi.RGN_METHOD_BINNED}
+
+
+def sync_spectrum(plan: SyncPlan, u, b, w=None, e=None, out=None, method="auto") -> torch.Tensor:
     n = u.shape[1]
     if out is None:
         out = torch.empty(plan.nbins, dtype=torch.float32, device=u.device)
-    check(lib().rgn_sync_spectrum(plan.handle, _soa(u, "u"), _soa(e, "e", n) or ptr3(None, None, None),
-                                  _soa(b, "b", n), _vec(w, "w", n), n, out.data_ptr(), _stream()))
+    check(lib().rgn_sync_spectrum_method(plan.handle, _soa(u, "u"),
+                                         _soa(e, "e", n) or ptr3(None, None, None),
+                                         _soa(b, "b", n), _vec(w, "w", n), n, METHODS[method],
+                                         out.data_ptr(), _stream()))
     return out
 
 
-def sync_adjoint(plan: SyncPlan, u, b, w, e, gbar, need=("w", "u", "b", "e")):
+def sync_adjoint(plan: SyncPlan, u, b, w, e, gbar, need=("w", "u", "b", "e"), method="auto"):
     n = u.shape[1]
     dev = u.device
     gw = torch.empty(n, dtype=torch.float32, device=dev) if "w" in need else None
@@ -77,9 +83,10 @@ def sync_adjoint(plan: SyncPlan, u, b, w, e, gbar, need=("w", "u", "b", "e")):
     ge = torch.empty(3, n, dtype=torch.float32, device=dev) if "e" in need else None
     gbar = gbar.contiguous().float()
     none3 = ptr3(None, None, None)
-    check(lib().rgn_sync_adjoint(
+    check(lib().rgn_sync_adjoint_method(
         plan.handle, _soa(u, "u"), _soa(e, "e", n) or none3, _soa(b, "b", n), _vec(w, "w", n), n,
-        _vec(gbar, "gbar", plan.nbins), gw.data_ptr() if gw is not None else None,
+        _vec(gbar, "gbar", plan.nbins), METHODS[method],
+        gw.data_ptr() if gw is not None else None,
         _soa(gu, "gu", n) or none3, _soa(gb, "gb", n) or none3, _soa(ge, "ge", n) or none3,
         _stream()))
     return gw, gu, gb, ge
