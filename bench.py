@@ -175,6 +175,57 @@ def bench_contraction(T, workloads, orc, torch, table):
     return res
 
 
+def stage_profile(T, plan, U, B, out, torch, reps):
+    """Average per-launch times (HIP events on the launch stream) of `reps`
+    default-method forward calls: kernels sorted by time, longest first."""
+    acc = {}
+    order = []
+    for _ in range(3):
+        T.spectrum_stages(plan, U, B, out=out)
+    for _ in range(reps):
+        for name, t in T.spectrum_stages(plan, U, B, out=out):
+            if name not in acc:
+                acc[name] = 0.0
+                order.append(name)
+            acc[name] += t
+    kernels = sorted(((k, acc[k] / reps) for k in order), key=lambda kv: -kv[1])
+    return {"kernels": kernels, "total_ms": sum(v for _, v in kernels)}
+
+
+def roofline(prof, n, nb, plan):
+    """Roofline of the step's longest kernel.
+
+    forward_kernel (direct): 6 FP32 VALU flop per particle x bin eval (the
+      inner loop: two FMAs forming u, one table-index conversion, the FMA of
+      the interpolant and the accumulating FMA); bound: FP32 VALU.
+    binned_bucket: reads u and B (24 B/particle), writes each contributing
+      source's (W, phi) (8 B/particle) and its block's per-bucket run table
+      (2 x NK x blocks x 4 B); bound: HBM.
+    binned_subhist: reads the sorted (W, phi) (8 B/particle) and writes its
+      per-item moments (2 x (M + 1) x 8 B per item); bound: HBM.
+    """
+    name, ms = prof["kernels"][0]
+    t = ms * 1e-3
+    info = plan.info()
+    if name == "forward_kernel":
+        achieved = FLOP_PER_EVAL_FWD * n * nb / t
+        return {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
+                "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": read_traffic(name),
+                "kernel": name, "kernel_ms": ms, "flop_per_eval": FLOP_PER_EVAL_FWD}
+    blocks = -(-n // 4096)
+    if name == "binned_bucket":
+        nbytes = 32 * n + 2 * 4 * blocks * info["NK"]
+    elif name == "binned_subhist":
+        items = n // 16384 + info["NK"]
+        nbytes = 8 * n + 16 * (info["M"] + 1) * items
+    else:
+        nbytes = 0
+    achieved = nbytes / t
+    return {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+            "frac": achieved / PEAK_HBM, "traffic": read_traffic(name), "kernel": name,
+            "kernel_ms": ms, "algorithmic_bytes": nbytes}
+
+
 def cpu_baseline(u, b, bins, target_cpu_seconds):
     """The oracle's faithful restatement (per-thread float accumulators, FP64
     per-pair OmegaSync, log10f interpolation), on a bounded sample."""
@@ -226,14 +277,21 @@ def main():
     table = T.default_table()
     plan = T.SyncPlan(table, bins)
     out = torch.empty(nb, device=dev)
+    evals = n * nb
+
+    # both methods, then the step the library runs by default (method "auto")
+    method_ms = {}
+    for m in ("direct", "binned"):
+        method_ms[m] = event_time(lambda: T.sync_spectrum(plan, U, B, out=out, method=m),
+                                  max(5, args.steps // 4), args.warmup, torch)
 
     def step():
         T.sync_spectrum(plan, U, B, out=out)
 
     ms = event_time(step, args.steps, args.warmup, torch)
-    evals = n * nb
     value = evals / (ms * 1e-3)
-    achieved = FLOP_PER_EVAL_FWD * evals / (ms * 1e-3)
+    auto = stage_profile(T, plan, U, B, out, torch, reps=20)
+    auto_method = "binned" if auto["kernels"][0][0].startswith("binned") else "direct"
 
     # parity spot check of the timed output against the oracle on a subset
     sub = 1 << 14
@@ -241,6 +299,9 @@ def main():
     _, ref = orc.sync_spectrum(u_h[:, :sub], b_h[:, :sub], bins, want_fp64=True)
     live = ref > 1e-12 * ref.max()
     parity = float(np.max(np.abs(S_sub[live] / ref[live] - 1)))
+    S_full = {m: T.sync_spectrum(plan, U, B, method=m).double().cpu().numpy() for m in ("direct", "binned")}
+    live = S_full["direct"] > 1e-12 * S_full["direct"].max()
+    methods_agree = float(np.max(np.abs(S_full["binned"][live] / S_full["direct"][live] - 1)))
 
     res = {
         "metric": "synchrotron particle x bin evals/s (fwd; fwd+adjoint) at <=1e-5 rel err, 1 GPU",
@@ -258,13 +319,11 @@ def main():
         "config": {"workload": f"SynchrotronSpectrum_3D forward, {n} particles x {nb} log photon "
                                "bins [1e-2, 1e7] mc^2, |u| power law p=-2.5 on [1,1e3], "
                                "B=(0,0,1)+N(0,0.3^2), unit weights",
-                   "particles": n, "photon_bins": nb},
-        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
-                     "traffic": read_traffic("forward_kernel"),
-                     "algorithmic_bytes": 24 * n,
-                     "kernel": "forward_kernel (+ the fixed-order combine of the same step)",
-                     "flop_per_eval": FLOP_PER_EVAL_FWD},
+                   "particles": n, "photon_bins": nb, "method": auto_method},
+        "roofline": roofline(auto, n, nb, plan),
+        "methods_ms": method_ms,
+        "methods_max_rel_diff": methods_agree,
+        "stages_ms": {name: t for name, t in auto["kernels"]},
         "parity_max_rel_err_subset": parity,
     }
 
@@ -281,11 +340,13 @@ def main():
             gbar = torch.where(S > 0, 2.0 * r / S.clamp_min(1e-30), torch.zeros_like(S))
             T.sync_adjoint(plan, U, B, w, None, gbar, need=("w", "u", "b"))
 
-        ms_fa = event_time(fwd_adj, max(3, args.steps // 2), 1, torch)
+        ms_fa = event_time(fwd_adj, max(3, args.steps // 2), 3, torch)
+        gones = torch.ones(nb, device=dev)
+        adj_ms = {m: event_time(lambda: T.sync_adjoint(plan, U, B, w, None, gones, need=("w", "u", "b"),
+                                                       method=m), 10, 3, torch)
+                  for m in ("direct", "binned")}
         extras["fwd_adj"] = {"ms": ms_fa, "evals_per_s": evals / (ms_fa * 1e-3),
-                             "ratio_to_fwd": ms_fa / ms,
-                             "frac_fp32_peak": (FLOP_PER_EVAL_FWD + FLOP_PER_EVAL_ADJ) * evals /
-                                               (ms_fa * 1e-3) / PEAK_FP32}
+                             "ratio_to_fwd": ms_fa / ms, "adjoint_ms_by_method": adj_ms}
         # energy histogram (HBM bound: 12 B per particle)
         hbins = orc.logspace(1e-2, 1e3, 200)
         hplan = T.HistPlan(hbins, True)

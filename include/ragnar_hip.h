@@ -138,11 +138,31 @@ int rgn_sync_cells(const rgn_sync_plan* plan, const float* const d_u[3],
 
 /* Diagnostics (no reference counterpart): per-particle (e_peak, chiR, W,
  * V_hi, V_lo) as the kernels see them (d_out: 5 * n floats), and the plan's
- * integer (Z, TP, NB) and double (K, Amin, Amax) scalars. */
+ * integer (Z, TP, NB, M, NK, binned method available) and double (K, Amin,
+ * Amax) scalars. */
 int rgn_sync_particle_terms(const rgn_sync_plan* plan, const float* const d_u[3],
                             const float* const d_e[3], const float* const d_b[3],
                             const float* d_w, size_t n, float* d_out, rgn_stream_t stream);
-int rgn_sync_plan_info(const rgn_sync_plan* plan, int* ints3, double* dbl3);
+int rgn_sync_plan_info(const rgn_sync_plan* plan, int* ints6, double* dbl3);
+
+/* Diagnostics (no reference counterpart): rgn_sync_spectrum_method /
+ * rgn_sync_adjoint_method with a HIP event recorded on `stream` before the
+ * first and after every kernel launch of the call.  On return (after waiting
+ * for the last event) stage_ms[i] holds the elapsed milliseconds of launch i,
+ * names[i] (if names is not NULL) a static string naming its kernel, and
+ * *nstages the number of launches (at most 16; max_stages entries are
+ * written).  The outputs are those of the untimed call. */
+int rgn_sync_spectrum_stages(const rgn_sync_plan* plan, const float* const d_u[3],
+                             const float* const d_e[3], const float* const d_b[3],
+                             const float* d_w, size_t n, int method, float* d_out,
+                             rgn_stream_t stream, float* stage_ms, const char** names,
+                             int max_stages, int* nstages);
+int rgn_sync_adjoint_stages(const rgn_sync_plan* plan, const float* const d_u[3],
+                            const float* const d_e[3], const float* const d_b[3],
+                            const float* d_w, size_t n, const float* d_gbar, int method,
+                            float* d_gw, float* const d_gu[3], float* const d_gb[3],
+                            float* const d_ge[3], rgn_stream_t stream, float* stage_ms,
+                            const char** names, int max_stages, int* nstages);
 
 /* ---- spectrum from a tabulated distribution ------------------------------
  * Replaces SynchrotronSpectrumFromDist, synchrotron.cpp:69-105, with

@@ -49,6 +49,10 @@ _SIGNATURES = {
     "rgn_sync_cells": (C.c_int, [vp, fptr3, fptr3, fptr3, vp, vp, C.c_int, vp, vp]),
     "rgn_sync_particle_terms": (C.c_int, [vp, fptr3, fptr3, fptr3, vp, sz, vp, vp]),
     "rgn_sync_plan_info": (C.c_int, [vp, vp, vp]),
+    "rgn_sync_spectrum_stages": (C.c_int, [vp, fptr3, fptr3, fptr3, vp, sz, C.c_int, vp, vp, vp,
+                                           vp, C.c_int, C.POINTER(C.c_int)]),
+    "rgn_sync_adjoint_stages": (C.c_int, [vp, fptr3, fptr3, fptr3, vp, sz, vp, C.c_int, vp, fptr3,
+                                          fptr3, fptr3, vp, vp, vp, C.c_int, C.POINTER(C.c_int)]),
     "rgn_sync_fromdist": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, vp, vp]),
     "rgn_sync_fromdist_batched": (C.c_int, [vp, vp, C.c_int, C.c_int, vp, sz, C.c_int, vp, vp]),
     "rgn_hist_plan_create": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(vp)]),
@@ -179,11 +183,11 @@ class SyncPlan:
         self.nbins = len(self.bins)
 
     def info(self):
-        ints = (C.c_int * 3)()
+        ints = (C.c_int * 6)()
         dbl = (C.c_double * 3)()
         check(lib().rgn_sync_plan_info(self.handle, ints, dbl))
-        return {"Z": ints[0], "TP": ints[1], "NB": ints[2], "K": dbl[0], "Amin": dbl[1],
-                "Amax": dbl[2]}
+        return {"Z": ints[0], "TP": ints[1], "NB": ints[2], "M": ints[3], "NK": ints[4],
+                "binned": bool(ints[5]), "K": dbl[0], "Amin": dbl[1], "Amax": dbl[2]}
 
     def __del__(self):
         if getattr(self, "handle", None) and _lib is not None:

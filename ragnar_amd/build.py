@@ -95,7 +95,10 @@ def build(jobs: int | None = None, verbose: bool = False) -> None:
             f"-L{HERE}", "-lragnar_hip", "-Wl,-rpath,$ORIGIN",
             "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"]
     if _hdf5_available():
-        link += [f"-L{HDF5_ROOT / 'lib'}", "-lhdf5", f"-Wl,-rpath,{HDF5_ROOT / 'lib'}"]
+        # the system runtime directory goes first so that the HDF5 prefix's own
+        # (older) libstdc++ is never picked up through this search path
+        link += [f"-L{HDF5_ROOT / 'lib'}", "-lhdf5", "-Wl,-rpath,/usr/lib/x86_64-linux-gnu",
+                 f"-Wl,-rpath,{HDF5_ROOT / 'lib'}"]
     _run(link)
     os.replace(tmp, mod)
     if verbose:

@@ -14,11 +14,20 @@
 // with P = prefix at r_e and T = totals.  This is the same piecewise-linear
 // function the direct kernel evaluates pair by pair, summed in FP64.
 //
-// Forward (a scatter): the sources are partitioned by k with a counting sort
-// (count, scan, scatter) and each bucket is reduced into its M + 1 sub-cells in
-// LDS; no global atomics (device-scope FP64 atomics run at ~14 G/s on MI355X).
+// Forward (a scatter), no global atomics (device-scope FP64 atomics run at
+// ~14 G/s on MI355X):
+//   bucket    each block of 4096 particles computes (k, W, phi), counting-sorts
+//             its sources by k in LDS and writes the sorted chunk coalesced;
+//             per (bucket, block) run lengths and offsets go to [k][block] tables;
+//   scan      per bucket, the exclusive scan of its run lengths over blocks;
+//             then work items of up to kItem sources per bucket;
+//   subhist   each item gathers its runs (a virtual range of the bucket) and
+//             reduces them into M + 1 sub-cells in LDS (ds_add_f64);
+//   prefix    per bucket: the items in order, then prefix sums over sub-cells;
+//   evaluate  one block per photon bin, fixed-order reduction over k.
 // Adjoint (a gather): dL/dS is folded into a table U[k][j] = (sum_e g'_e C,
-// sum_e g'_e sigma) and every particle reads one entry.
+// sum_e g'_e sigma), built per k as a base plus prefix sums of per-rank
+// switches from segment m to m - 1, and every particle reads one entry.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -38,6 +47,7 @@ void build_binned_tables(SyncPlan& plan, const std::vector<float>& xs, const std
   alphas.erase(std::unique(alphas.begin(), alphas.end()), alphas.end());
   plan.M = static_cast<int>(alphas.size());
   std::vector<BinInfo> bi(plan.nbins);
+  std::vector<int> rcount(plan.M + 1, 0);
   for (int e = 0; e < plan.nbins; ++e) {
     if (!plan.live[e]) {
       bi[e] = BinInfo{0.0, 0, -1};
@@ -47,6 +57,17 @@ void build_binned_tables(SyncPlan& plan, const std::vector<float>& xs, const std
     const double al = plan.A[e] - a;
     const int r = static_cast<int>(std::lower_bound(alphas.begin(), alphas.end(), al) - alphas.begin());
     bi[e] = BinInfo{plan.A[e], static_cast<int>(a), r};
+    ++rcount[r];
+  }
+  // live bins grouped by alpha rank, ascending e within a rank
+  std::vector<int> rptr(plan.M + 1, 0), rlist;
+  for (int r = 0; r < plan.M; ++r) rptr[r + 1] = rptr[r] + rcount[r];
+  rlist.resize(std::max(1, rptr[plan.M]));
+  {
+    std::vector<int> cur(rptr.begin(), rptr.end() - 1);
+    for (int e = 0; e < plan.nbins; ++e) {
+      if (bi[e].r >= 0) rlist[cur[bi[e].r]++] = e;
+    }
   }
   std::vector<SegInfo> seg(std::max(T - 1, 1));
   auto tau = [&](int i) { return (std::log2(static_cast<double>(xs[i])) - plan.log2_xmin) * plan.K; };
@@ -63,10 +84,27 @@ void build_binned_tables(SyncPlan& plan, const std::vector<float>& xs, const std
     plan.kmin = 0;
     plan.NK = 0;
   }
+  // sub-cell lookup grid: jlut[q] = number of alphas below q / JQ
+  int JQ = 64;
+  while (JQ < 2 * plan.M && JQ < 8192) JQ *= 2;
+  plan.JQ = JQ;
+  std::vector<unsigned short> jlut(JQ);
+  for (int q = 0; q < JQ; ++q) {
+    const double edge = static_cast<double>(q) / JQ;
+    jlut[q] = static_cast<unsigned short>(std::lower_bound(alphas.begin(), alphas.end(), edge) - alphas.begin());
+  }
+  plan.binned_ok = plan.M <= kBinnedMaxM && plan.NK <= kBinnedMaxNK;
+
   plan.alpha.reset(std::max<size_t>(1, alphas.size()) * sizeof(double));
   if (!alphas.empty()) {
     RGN_HIP(hipMemcpy(plan.alpha.ptr, alphas.data(), alphas.size() * sizeof(double), hipMemcpyHostToDevice));
   }
+  plan.jlut.reset(jlut.size() * sizeof(unsigned short));
+  RGN_HIP(hipMemcpy(plan.jlut.ptr, jlut.data(), plan.jlut.bytes, hipMemcpyHostToDevice));
+  plan.rptr.reset(rptr.size() * sizeof(int));
+  RGN_HIP(hipMemcpy(plan.rptr.ptr, rptr.data(), plan.rptr.bytes, hipMemcpyHostToDevice));
+  plan.rlist.reset(rlist.size() * sizeof(int));
+  RGN_HIP(hipMemcpy(plan.rlist.ptr, rlist.data(), plan.rlist.bytes, hipMemcpyHostToDevice));
   plan.binfo.reset(bi.size() * sizeof(BinInfo));
   RGN_HIP(hipMemcpy(plan.binfo.ptr, bi.data(), bi.size() * sizeof(BinInfo), hipMemcpyHostToDevice));
   plan.seg.reset(seg.size() * sizeof(SegInfo));
@@ -76,224 +114,370 @@ void build_binned_tables(SyncPlan& plan, const std::vector<float>& xs, const std
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kItem = 4096;  // sources per sub-histogram work item
+constexpr int kPer = 16;                   // particles per thread in the bucket pass
+constexpr int kChunk = kThreads * kPer;    // particles per bucket block
+constexpr unsigned kItem = 16384;          // sources per sub-histogram work item
+constexpr int kSubThreads = 512;           // threads of a sub-histogram work item
+constexpr int kWin = 2048;                 // runs per LDS window of a work item
+constexpr int kPrefThreads = 1024;         // 4 item groups x 256 sub-cells per prefix block
 
 struct BinnedDev {
   const double* alpha;
+  const unsigned short* jlut;
+  const int* rptr;
+  const int* rlist;
   const BinInfo* binfo;
   const SegInfo* seg;
   const float* bine;
-  int M, T, kmin, NK, nbins;
-  float B0, g_syn, e_at;
+  int M, JQ, T, kmin, NK, nbins;
+  EmitConst em;
   double K;
 };
 
 BinnedDev binned_view(const SyncPlan& p) {
-  return BinnedDev{p.alpha.as<double>(), p.binfo.as<BinInfo>(), p.seg.as<SegInfo>(), p.bine.as<float>(),
-                   p.M, p.T, p.kmin, p.NK, p.nbins, p.B0, p.g_syn, p.e_at, p.K};
+  return BinnedDev{p.alpha.as<double>(), p.jlut.as<unsigned short>(), p.rptr.as<int>(), p.rlist.as<int>(),
+                   p.binfo.as<BinInfo>(), p.seg.as<SegInfo>(), p.bine.as<float>(),
+                   p.M, p.JQ, p.T, p.kmin, p.NK, p.nbins, emit_const(p), p.K};
 }
 
-// W and B of a particle; false when it contributes nowhere.
-__device__ __forceinline__ bool particle_wb(const RawParticle& r, const BinnedDev& P, float& W, double& B) {
-  float e_peak, chiR;
-  omega_sync(r.ux, r.uy, r.uz, r.ex, r.ey, r.ez, r.bx, r.by, r.bz, P.B0, P.g_syn, P.e_at, e_peak, chiR);
-  W = chiR * r.w;
-  if (!(e_peak > 0.0f)) return false;
-  B = log2(static_cast<double>(e_peak)) * P.K;
+// W and B of a particle from its OmegaSync terms; false when it contributes
+// nowhere.
+__device__ __forceinline__ bool particle_wb(const OmegaTerms& o, const RawParticle& r, const BinnedDev& P,
+                                            float& W, double& B) {
+  W = o.chiR * r.w;
+  if (!(o.e_peak > 0.0f && o.e_peak < INFINITY)) return false;  // synchrotron.hpp:162; NaN fails too
+  B = log2_pos(o.e_peak) * P.K;
   return B >= static_cast<double>(P.kmin) && B < static_cast<double>(P.kmin + P.NK);
 }
 
-// Number of alphas strictly below phi (binary search in LDS).
-__device__ __forceinline__ int subcell(const double* __restrict__ alpha, int M, double phi) {
-  int lo = 0, hi = M;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (alpha[mid] < phi) lo = mid + 1; else hi = mid;
-  }
-  return lo;
+// Number of alphas strictly below phi (0 <= phi < 1): the lookup grid gives
+// the count below the grid cell's left edge, a short scan finishes it.
+__device__ __forceinline__ int subcell(const double* __restrict__ alpha, const unsigned short* __restrict__ jlut,
+                                       int M, int JQ, double phi) {
+  int j = jlut[min(static_cast<int>(phi * JQ), JQ - 1)];
+  while (j < M && alpha[j] < phi) ++j;
+  return j;
 }
 
-// Pass 1: per particle (W, phi, k) records and per-block bucket counts.
-__global__ void __launch_bounds__(kThreads) binned_count(ParticlesDev D, size_t n, size_t Q, BinnedDev P,
-                                                         float* __restrict__ recW, float* __restrict__ recPhi,
-                                                         int* __restrict__ recK, unsigned* __restrict__ counts) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  unsigned* cnt = reinterpret_cast<unsigned*>(smem);
-  for (int i = threadIdx.x; i < P.NK; i += kThreads) cnt[i] = 0u;
-  __syncthreads();
-  const size_t p0 = static_cast<size_t>(blockIdx.x) * Q;
-  const size_t p1 = min(n, p0 + Q);
-  size_t p = p0 + threadIdx.x;
-  RawParticle raw{};
-  if (p < p1) raw = load_particle(D, p);
-  for (; p < p1; p += kThreads) {
-    const RawParticle cur = raw;
-    if (p + kThreads < p1) raw = load_particle(D, p + kThreads);
-    float W;
-    double B;
-    int k = -1;
-    float phi = 0.0f;
-    if (particle_wb(cur, P, W, B)) {
-      const double fl = floor(B);
-      k = static_cast<int>(fl) - P.kmin;
-      phi = fminf(static_cast<float>(B - fl), 0x1.fffffep-1f);
-      atomicAdd(cnt + k, 1u);
-    }
-    recW[p] = W;
-    recPhi[p] = phi;
-    recK[p] = k;
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < P.NK; i += kThreads) counts[static_cast<size_t>(i) * gridDim.x + blockIdx.x] = cnt[i];
-}
-
-// Pass 2a: exclusive scan of each bucket's per-block counts (one block per bucket).
-__global__ void __launch_bounds__(kThreads) binned_scan_rows(unsigned* __restrict__ counts, int G,
-                                                             unsigned* __restrict__ totals) {
-  const int k = blockIdx.x;
-  unsigned* row = counts + static_cast<size_t>(k) * G;
-  __shared__ unsigned part[kThreads];
-  // each thread owns a contiguous run of the row
-  const int per = (G + kThreads - 1) / kThreads;
-  const int b0 = threadIdx.x * per, b1 = min(G, b0 + per);
-  unsigned s = 0;
-  for (int b = b0; b < b1; ++b) s += row[b];
+// Block-wide exclusive scan of a[0..N) into o[0..N) (o may alias a); returns
+// the total.  part: NT words of LDS.  Every thread of the NT-thread block calls it.
+template <class T, int NT = kThreads, class PA, class PO>
+__device__ T block_scan_exclusive(PA a, PO o, int N, T* part) {
+  const int per = (N + NT - 1) / NT;
+  const int i0 = min(N, static_cast<int>(threadIdx.x) * per), i1 = min(N, i0 + per);
+  T s = 0;
+  for (int i = i0; i < i1; ++i) s += a[i];
   part[threadIdx.x] = s;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned acc = 0;
-    for (int i = 0; i < kThreads; ++i) {
-      const unsigned v = part[i];
-      part[i] = acc;
-      acc += v;
-    }
-    totals[k] = acc;
+  for (int d = 1; d < NT; d <<= 1) {
+    const T v = static_cast<int>(threadIdx.x) >= d ? part[threadIdx.x - d] : T(0);
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
   }
-  __syncthreads();
-  unsigned acc = part[threadIdx.x];
-  for (int b = b0; b < b1; ++b) {
-    const unsigned v = row[b];
-    row[b] = acc;
+  T acc = threadIdx.x ? part[threadIdx.x - 1] : T(0);
+  const T total = part[NT - 1];
+  for (int i = i0; i < i1; ++i) {
+    const T v = a[i];
+    o[i] = acc;
     acc += v;
   }
+  __syncthreads();
+  return total;
 }
 
-// Pass 2b: bucket starts and sub-histogram work items (one block).
-__global__ void __launch_bounds__(kThreads) binned_scan_totals(const unsigned* __restrict__ totals, int NK,
-                                                               unsigned* __restrict__ bstart,
-                                                               unsigned* __restrict__ istart) {
-  if (threadIdx.x == 0) {
-    unsigned s = 0, it = 0;
-    for (int k = 0; k < NK; ++k) {
-      bstart[k] = s;
-      istart[k] = it;
-      s += totals[k];
-      it += (totals[k] + kItem - 1) / kItem;
-    }
-    bstart[NK] = s;
-    istart[NK] = it;
-  }
-}
-
-// Pass 3: scatter (W, phi) into bucket order.
-__global__ void __launch_bounds__(kThreads) binned_scatter(size_t n, size_t Q, int NK,
-                                                           const float* __restrict__ recW,
-                                                           const float* __restrict__ recPhi,
-                                                           const int* __restrict__ recK,
-                                                           const unsigned* __restrict__ counts,
-                                                           const unsigned* __restrict__ bstart,
-                                                           float* __restrict__ sortW, float* __restrict__ sortPhi) {
+// Pass 1: per block of kChunk particles, sources sorted by bucket k.
+//   sortW/sortPhi[b * kChunk + i]: the block's sources in bucket order;
+//   cnt[k * G + b], loff[k * G + b]: run length and start of bucket k in block b.
+__global__ void __launch_bounds__(kThreads) binned_bucket(ParticlesDev D, unsigned n, BinnedDev P,
+                                                          float* __restrict__ sortW, float* __restrict__ sortPhi,
+                                                          unsigned* __restrict__ cnt, unsigned* __restrict__ loff) {
   extern __shared__ __align__(16) unsigned char smem[];
-  unsigned* cur = reinterpret_cast<unsigned*>(smem);
-  for (int i = threadIdx.x; i < NK; i += kThreads) {
-    cur[i] = bstart[i] + counts[static_cast<size_t>(i) * gridDim.x + blockIdx.x];
+  const int NK = P.NK;
+  unsigned* hist = reinterpret_cast<unsigned*>(smem);
+  unsigned* off = hist + NK;
+  unsigned* part = off + NK;
+  float* sW = reinterpret_cast<float*>(part + kThreads);
+  float* sP = sW + kChunk;
+  for (int i = threadIdx.x; i < NK; i += kThreads) hist[i] = 0u;
+  __syncthreads();
+  const unsigned G = gridDim.x;
+  const unsigned base = blockIdx.x * static_cast<unsigned>(kChunk);
+  int kk[kPer];
+  float Wr[kPer], ph[kPer];
+  unsigned rk[kPer];
+  unsigned p = base + threadIdx.x;
+  RawParticle raw{};
+  if (p < n) raw = load_particle(D, p);
+#pragma unroll
+  for (int i = 0; i < kPer; ++i, p += kThreads) {
+    const RawParticle cur = raw;
+    if (i + 1 < kPer && p + kThreads < n) raw = load_particle(D, p + kThreads);
+    kk[i] = -1;
+    Wr[i] = 0.0f;
+    ph[i] = 0.0f;
+    rk[i] = 0u;
+    float W;
+    double B;
+    if (p < n && particle_wb(omega_terms(cur, P.em), cur, P, W, B)) {
+      const double fl = floor(B);
+      kk[i] = static_cast<int>(fl) - P.kmin;
+      Wr[i] = W;
+      ph[i] = fminf(static_cast<float>(B - fl), 0x1.fffffep-1f);
+      rk[i] = atomicAdd(hist + kk[i], 1u);
+    }
   }
   __syncthreads();
-  const size_t p0 = static_cast<size_t>(blockIdx.x) * Q;
-  const size_t p1 = min(n, p0 + Q);
-  for (size_t p = p0 + threadIdx.x; p < p1; p += kThreads) {
-    const int k = recK[p];
-    if (k < 0) continue;
-    const unsigned pos = atomicAdd(cur + k, 1u);
-    sortW[pos] = recW[p];
-    sortPhi[pos] = recPhi[p];
+  block_scan_exclusive<unsigned>(hist, off, NK, part);
+  for (int k = threadIdx.x; k < NK; k += kThreads) {
+    cnt[static_cast<size_t>(k) * G + blockIdx.x] = hist[k];
+    loff[static_cast<size_t>(k) * G + blockIdx.x] = off[k];
+  }
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) {
+    if (kk[i] >= 0) {
+      const unsigned pos = off[kk[i]] + rk[i];
+      sW[pos] = Wr[i];
+      sP[pos] = ph[i];
+    }
+  }
+  __syncthreads();
+  const unsigned total = off[NK - 1] + hist[NK - 1];
+  for (unsigned i = threadIdx.x; i < total; i += kThreads) {
+    sortW[base + i] = sW[i];
+    sortPhi[base + i] = sP[i];
   }
 }
 
-// Pass 4: sub-histogram of one work item (a chunk of one bucket).
-__global__ void __launch_bounds__(kThreads) binned_subhist(BinnedDev P, const unsigned* __restrict__ bstart,
-                                                           const unsigned* __restrict__ istart,
-                                                           const float* __restrict__ sortW,
-                                                           const float* __restrict__ sortPhi,
-                                                           double* __restrict__ part) {
+// Pass 2a: per bucket k, rowoff[k * (G + 1) + b] = sum of cnt[k][b' < b];
+// rowoff[k * (G + 1) + G] = totals[k].
+__global__ void __launch_bounds__(kThreads) binned_scan_rows(const unsigned* __restrict__ cnt, int G,
+                                                             unsigned* __restrict__ rowoff,
+                                                             unsigned* __restrict__ totals) {
+  __shared__ unsigned part[kThreads];
+  const int k = blockIdx.x;
+  unsigned* row = rowoff + static_cast<size_t>(k) * (G + 1);
+  const unsigned total = block_scan_exclusive<unsigned>(cnt + static_cast<size_t>(k) * G, row, G, part);
+  if (threadIdx.x == 0) {
+    row[G] = total;
+    totals[k] = total;
+  }
+}
+
+// Pass 2b: work items per bucket (one block): istart[k] = first item of bucket
+// k, istart[NK] = number of items.
+__global__ void __launch_bounds__(kThreads) binned_scan_items(const unsigned* __restrict__ totals, int NK,
+                                                              unsigned* __restrict__ istart) {
+  __shared__ unsigned items[kBinnedMaxNK];
+  __shared__ unsigned part[kThreads];
+  for (int k = threadIdx.x; k < NK; k += kThreads) items[k] = (totals[k] + kItem - 1) / kItem;
+  __syncthreads();
+  const unsigned total = block_scan_exclusive<unsigned>(items, istart, NK, part);
+  if (threadIdx.x == 0) istart[NK] = total;
+}
+
+// Pass 3: sub-histogram of one work item: sources [v0, v1) of bucket k, found
+// as runs of the bucket blocks' sorted chunks.  Each wave takes the non-empty
+// runs of 64 consecutive blocks at a time (ballot) and reads every run
+// coalesced, four loads in flight per lane.
+__global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G, const unsigned* __restrict__ istart,
+                                                              const unsigned* __restrict__ totals,
+                                                              const unsigned* __restrict__ rowoff,
+                                                              const unsigned* __restrict__ loff,
+                                                              const float* __restrict__ sortW,
+                                                              const float* __restrict__ sortPhi,
+                                                              double* __restrict__ partial) {
   extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int kWaves = kSubThreads / 64;
   const int M1 = P.M + 1;
   double* m0 = reinterpret_cast<double*>(smem);
   double* m1 = m0 + M1;
   double* alpha = m1 + M1;
+  unsigned* wro = reinterpret_cast<unsigned*>(alpha + P.M);  // kWin + 1
+  unsigned* wlo = wro + kWin + 1;                              // kWin
+  unsigned short* jlut = reinterpret_cast<unsigned short*>(wlo + kWin);
+  __shared__ unsigned s_cnt[3];
+  __shared__ unsigned s_part[kSubThreads];
   const unsigned it = blockIdx.x;
   if (it >= istart[P.NK]) return;  // uniform per block
-  for (int i = threadIdx.x; i < M1; i += kThreads) {
+  if (threadIdx.x < 3) s_cnt[threadIdx.x] = 0u;
+  for (int i = threadIdx.x; i < M1; i += kSubThreads) {
     m0[i] = 0.0;
     m1[i] = 0.0;
   }
-  for (int i = threadIdx.x; i < P.M; i += kThreads) alpha[i] = P.alpha[i];
-  int lo = 0, hi = P.NK - 1;  // bucket of this item: last k with istart[k] <= it
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (istart[mid] <= it) lo = mid; else hi = mid - 1;
-  }
-  const int k = lo;
-  const unsigned s0 = bstart[k] + (it - istart[k]) * kItem;
-  const unsigned s1 = min(bstart[k + 1], s0 + kItem);
+  for (int i = threadIdx.x; i < P.M; i += kSubThreads) alpha[i] = P.alpha[i];
+  for (int i = threadIdx.x; i < P.JQ; i += kSubThreads) jlut[i] = P.jlut[i];
   __syncthreads();
-  for (unsigned s = s0 + threadIdx.x; s < s1; s += kThreads) {
-    const double w = sortW[s];
-    const double phi = sortPhi[s];
-    const int j = subcell(alpha, P.M, phi);
-    atomicAdd(m0 + j, w);
-    atomicAdd(m1 + j, w * phi);
+  // Monotone searches as counts (all loads of a search in flight at once):
+  // bucket of this item k = #{k' : istart[k'] <= it} - 1.
+  {
+    unsigned c = 0;
+    for (int q = threadIdx.x; q < P.NK; q += kSubThreads) c += istart[q] <= it;
+    if (c) atomicAdd(&s_cnt[0], c);
   }
   __syncthreads();
-  double* out = part + static_cast<size_t>(it) * 2 * M1;
-  for (int i = threadIdx.x; i < M1; i += kThreads) {
+  const int k = static_cast<int>(s_cnt[0]) - 1;
+  const unsigned v0 = (it - istart[k]) * kItem;
+  const unsigned v1 = min(totals[k], v0 + kItem);
+  const unsigned* row = rowoff + static_cast<size_t>(k) * (G + 1);
+  const unsigned* lrow = loff + static_cast<size_t>(k) * G;
+  // b_lo = #{b < G : row[b] <= v0} - 1, b_hi = #{b < G : row[b] <= v1 - 1} - 1
+  {
+    unsigned c0 = 0, c1 = 0;
+    for (int q = threadIdx.x; q < G; q += kSubThreads) {
+      const unsigned r = row[q];
+      c0 += r <= v0;
+      c1 += r <= v1 - 1;
+    }
+    if (c0) atomicAdd(&s_cnt[1], c0);
+    if (c1) atomicAdd(&s_cnt[2], c1);
+  }
+  __syncthreads();
+  const int blo = static_cast<int>(s_cnt[1]) - 1, bhi = static_cast<int>(s_cnt[2]) - 1;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int ws = blo; ws <= bhi; ws += kWin) {
+    const int nw = min(bhi - ws + 1, kWin);
+    // the window's runs clipped to [v0, v1), empty ones dropped:
+    // run c covers virtual [crs[c], crs[c + 1]) from physical index cidx[c]
+    constexpr int kPerT = kWin / kSubThreads;
+    unsigned fa[kPerT], fidx[kPerT];
+    unsigned nfl = 0;
+#pragma unroll
+    for (int q = 0; q < kPerT; ++q) {
+      const int i = threadIdx.x * kPerT + q;
+      fa[q] = 0u;
+      fidx[q] = ~0u;
+      if (i < nw) {
+        const unsigned r0 = row[ws + i];
+        const unsigned ra = max(r0, v0), rz = min(row[ws + i + 1], v1);
+        if (ra < rz) {
+          fa[q] = ra;
+          fidx[q] = static_cast<unsigned>(ws + i) * kChunk + lrow[ws + i] + (ra - r0);
+          ++nfl;
+        }
+      }
+    }
+    wro[threadIdx.x] = nfl;
+    __syncthreads();
+    const int nc = static_cast<int>(block_scan_exclusive<unsigned, kSubThreads>(wro, wro, kSubThreads, s_part));
+    unsigned c = wro[threadIdx.x];
+    __syncthreads();  // wro is rewritten below
+    unsigned* crs = wro;  // nc + 1 entries
+    unsigned* cidx = wlo;  // nc entries
+#pragma unroll
+    for (int q = 0; q < kPerT; ++q) {
+      if (fidx[q] != ~0u) {
+        crs[c] = fa[q];
+        cidx[c] = fidx[q];
+        ++c;
+      }
+    }
+    if (threadIdx.x == 0) crs[nc] = min(row[ws + nw], v1);
+    __syncthreads();
+    if (nc > 0) {
+      // wave w walks positions [pa, pb) of the window, lane-interleaved
+      const unsigned va = crs[0], vb = crs[nc];
+      const unsigned per = ((vb - va + kWaves - 1) / kWaves + 63) & ~63u;
+      const unsigned pa = min(vb, va + per * wave), pb = min(vb, pa + per);
+      int cur = 0;
+      if (pa + lane < pb) {
+        int lo = 0, hi = nc - 1;  // last run starting at or before the lane's first position
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (crs[mid] <= pa + lane) lo = mid; else hi = mid - 1;
+        }
+        cur = lo;
+      }
+      for (unsigned v00 = pa; v00 < pb; v00 += 256) {
+        unsigned idx[4];
+        bool ok[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const unsigned v = v00 + q * 64 + lane;
+          ok[q] = v < pb;
+          idx[q] = 0u;
+          if (ok[q]) {
+            if (crs[cur + 1] <= v) {
+              ++cur;
+              if (crs[cur + 1] <= v) {
+                int lo = cur + 1, hi = nc - 1;
+                while (lo < hi) {
+                  const int mid = (lo + hi + 1) >> 1;
+                  if (crs[mid] <= v) lo = mid; else hi = mid - 1;
+                }
+                cur = lo;
+              }
+            }
+            idx[q] = cidx[cur] + (v - crs[cur]);
+          }
+        }
+        float w[4], ph[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          w[q] = ok[q] ? sortW[idx[q]] : 0.0f;
+          ph[q] = ok[q] ? sortPhi[idx[q]] : 0.0f;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (ok[q]) {
+            const double wd = w[q], phd = ph[q];
+            const int j = subcell(alpha, jlut, P.M, P.JQ, phd);
+            atomicAdd(m0 + j, wd);
+            atomicAdd(m1 + j, wd * phd);
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  double* out = partial + static_cast<size_t>(it) * 2 * M1;
+  for (int i = threadIdx.x; i < M1; i += kSubThreads) {
     out[i] = m0[i];
     out[M1 + i] = m1[i];
   }
 }
 
-// Pass 5: per bucket, sum its items in order and take prefix sums over j.
-__global__ void __launch_bounds__(kThreads) binned_prefix(int M, const unsigned* __restrict__ istart,
-                                                          const double* __restrict__ part,
-                                                          double* __restrict__ P0, double* __restrict__ P1) {
+// Pass 4: per bucket, its items summed (four interleaved item groups, combined
+// in a fixed order), then inclusive prefix sums over sub-cells:
+// P0[k][j] = sum_{j' <= j} m0, P1 likewise.
+__global__ void __launch_bounds__(kPrefThreads) binned_prefix(int M, const unsigned* __restrict__ istart,
+                                                              const double* __restrict__ partial,
+                                                              double* __restrict__ P0, double* __restrict__ P1) {
   extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double part[kPrefThreads];
+  __shared__ double red[2][4][256];
   const int M1 = M + 1;
   double* s0 = reinterpret_cast<double*>(smem);
   double* s1 = s0 + M1;
   const int k = blockIdx.x;
-  for (int j = threadIdx.x; j < M1; j += kThreads) {
+  const unsigned i0 = istart[k], i1 = istart[k + 1];
+  const int grp = threadIdx.x >> 8, col = threadIdx.x & 255;
+  for (int j0 = 0; j0 < M1; j0 += 256) {
+    const int j = j0 + col;
     double a = 0.0, b = 0.0;
-    for (unsigned it = istart[k]; it < istart[k + 1]; ++it) {
-      a += part[static_cast<size_t>(it) * 2 * M1 + j];
-      b += part[static_cast<size_t>(it) * 2 * M1 + M1 + j];
+    if (j < M1) {
+      for (unsigned it = i0 + grp; it < i1; it += 4) {
+        a += partial[static_cast<size_t>(it) * 2 * M1 + j];
+        b += partial[static_cast<size_t>(it) * 2 * M1 + M1 + j];
+      }
     }
-    s0[j] = a;
-    s1[j] = b;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double a = 0.0, b = 0.0;
-    for (int j = 0; j < M1; ++j) {
-      a += s0[j];
-      b += s1[j];
-      s0[j] = a;
-      s1[j] = b;
+    red[0][grp][col] = a;
+    red[1][grp][col] = b;
+    __syncthreads();
+    if (grp == 0 && j < M1) {
+      s0[j] = ((red[0][0][col] + red[0][1][col]) + red[0][2][col]) + red[0][3][col];
+      s1[j] = ((red[1][0][col] + red[1][1][col]) + red[1][2][col]) + red[1][3][col];
     }
+    __syncthreads();
   }
-  __syncthreads();
-  for (int j = threadIdx.x; j < M1; j += kThreads) {
-    P0[static_cast<size_t>(k) * M1 + j] = s0[j];
-    P1[static_cast<size_t>(k) * M1 + j] = s1[j];
+  double* r0 = P0 + static_cast<size_t>(k) * M1;
+  double* r1 = P1 + static_cast<size_t>(k) * M1;
+  const double t0 = block_scan_exclusive<double, kPrefThreads>(s0, s0, M1, part);
+  const double t1 = block_scan_exclusive<double, kPrefThreads>(s1, s1, M1, part);
+  // inclusive = exclusive of the next entry; the last one is the total
+  for (int j = threadIdx.x; j < M1; j += kPrefThreads) {
+    r0[j] = j + 1 < M1 ? s0[j + 1] : t0;
+    r1[j] = j + 1 < M1 ? s1[j + 1] : t1;
   }
 }
 
@@ -309,7 +493,7 @@ __device__ __forceinline__ void seg_line(const BinnedDev& P, const BinInfo& b, i
   C = g.y + (b.A - k_abs - g.tau) * g.sigma;
 }
 
-// Pass 6: S_e = e_e sum_k (...), one block per bin, fixed-order reduction.
+// Pass 5: S_e = e_e sum_k (...), one block per bin, fixed-order reduction.
 __global__ void __launch_bounds__(kThreads) binned_evaluate(BinnedDev P, const double* __restrict__ P0,
                                                             const double* __restrict__ P1,
                                                             float* __restrict__ out) {
@@ -345,33 +529,55 @@ __global__ void __launch_bounds__(kThreads) binned_evaluate(BinnedDev P, const d
 }
 
 // ---------------------------------------------------------------------------
-// Adjoint: U[k][j] = (sum_e g'_e C_s, sum_e g'_e sigma_s), s = m or m - 1.
+// Adjoint.  A source in bucket k, sub-cell j sees bin e on segment m = a_e - k
+// when j <= r_e and on m - 1 otherwise, so
+//   U[k][j] = sum_e g'_e line_m(e, k) + sum_{r < j} D_k[r],
+//   D_k[r]  = sum_{e: r_e = r} g'_e (line_{m-1} - line_m)(e, k),
+// with line = (C, sigma) and g'_e = dL/dS_e * e_e.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kThreads) binned_adj_table(BinnedDev P, const float* __restrict__ gbar,
                                                              double2* __restrict__ U) {
   extern __shared__ __align__(16) unsigned char smem[];
-  BinInfo* bi = reinterpret_cast<BinInfo*>(smem);
-  double* gp = reinterpret_cast<double*>(bi + P.nbins);
-  for (int e = threadIdx.x; e < P.nbins; e += kThreads) {
-    bi[e] = P.binfo[e];
-    gp[e] = static_cast<double>(gbar[e]) * P.bine[e];
-  }
-  __syncthreads();
-  const int M1 = P.M + 1;
+  double* d0 = reinterpret_cast<double*>(smem);
+  double* d1 = d0 + P.M;
+  __shared__ double part[kThreads];
+  __shared__ double base0[kThreads], base1[kThreads];
   const int kk = blockIdx.x;
   const int k = P.kmin + kk;
-  for (int j = threadIdx.x; j < M1; j += kThreads) {
-    double u0 = 0.0, u1 = 0.0;
-    for (int e = 0; e < P.nbins; ++e) {
-      const BinInfo b = bi[e];
-      if (b.r < 0 || gp[e] == 0.0) continue;
-      const int s = (j <= b.r) ? b.a - k : b.a - k - 1;
-      double C, sig;
-      seg_line(P, b, k, s, C, sig);
-      u0 += gp[e] * C;
-      u1 += gp[e] * sig;
+  double b0 = 0.0, b1 = 0.0;
+  for (int r = threadIdx.x; r < P.M; r += kThreads) {
+    double x0 = 0.0, x1 = 0.0;
+    for (int q = P.rptr[r]; q < P.rptr[r + 1]; ++q) {
+      const int e = P.rlist[q];
+      const BinInfo b = P.binfo[e];
+      const double g = static_cast<double>(gbar[e]) * P.bine[e];
+      double Cp, sp, Cm, sm;
+      seg_line(P, b, k, b.a - k, Cp, sp);
+      seg_line(P, b, k, b.a - k - 1, Cm, sm);
+      b0 += g * Cp;
+      b1 += g * sp;
+      x0 += g * (Cm - Cp);
+      x1 += g * (sm - sp);
     }
-    U[static_cast<size_t>(kk) * M1 + j] = make_double2(u0, u1);
+    d0[r] = x0;
+    d1[r] = x1;
+  }
+  base0[threadIdx.x] = b0;
+  base1[threadIdx.x] = b1;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (static_cast<int>(threadIdx.x) < w) {
+      base0[threadIdx.x] += base0[threadIdx.x + w];
+      base1[threadIdx.x] += base1[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  const double B0 = base0[0], B1 = base1[0];
+  const double t0 = block_scan_exclusive<double>(d0, d0, P.M, part);
+  const double t1 = block_scan_exclusive<double>(d1, d1, P.M, part);
+  double2* row = U + static_cast<size_t>(kk) * (P.M + 1);
+  for (int j = threadIdx.x; j <= P.M; j += kThreads) {
+    row[j] = j < P.M ? make_double2(B0 + d0[j], B1 + d1[j]) : make_double2(B0 + t0, B1 + t1);
   }
 }
 
@@ -386,7 +592,9 @@ __global__ void __launch_bounds__(kThreads) binned_adj_gather(ParticlesDev D, si
                                                               const double2* __restrict__ U, GradPtrs G) {
   extern __shared__ __align__(16) unsigned char smem[];
   double* alpha = reinterpret_cast<double*>(smem);
+  unsigned short* jlut = reinterpret_cast<unsigned short*>(alpha + P.M);
   for (int i = threadIdx.x; i < P.M; i += kThreads) alpha[i] = P.alpha[i];
+  for (int i = threadIdx.x; i < P.JQ; i += kThreads) jlut[i] = P.jlut[i];
   __syncthreads();
   const double dlnx = P.K / 0.69314718055994530942;  // dt / dln x
   const size_t stride = static_cast<size_t>(gridDim.x) * kThreads;
@@ -398,44 +606,20 @@ __global__ void __launch_bounds__(kThreads) binned_adj_gather(ParticlesDev D, si
     if (p + stride < n) raw = load_particle(D, p + stride);
     float W;
     double B;
-    double gw = 0, gux = 0, guy = 0, guz = 0, gbx = 0, gby = 0, gbz = 0, gex = 0, gey = 0, gez = 0;
-    if (particle_wb(cur, P, W, B)) {
+    ParticleGrads g{};
+    const OmegaTerms o = omega_terms(cur, P.em);
+    if (particle_wb(o, cur, P, W, B)) {
       const double fl = floor(B);
       const double phi = B - fl;
       const int kk = static_cast<int>(fl) - P.kmin;
-      const int j = subcell(alpha, P.M, phi);
+      const int j = subcell(alpha, jlut, P.M, P.JQ, phi);
       const double2 u = U[static_cast<size_t>(kk) * (P.M + 1) + j];
-      const double A = u.x - u.y * phi;  // sum_e g'_e F
-      const double C = dlnx * u.y;        // sum_e g'_e dF/dln x
-      const double ux = cur.ux, uy = cur.uy, uz = cur.uz, ex = cur.ex, ey = cur.ey, ez = cur.ez;
-      const double bx = cur.bx, by = cur.by, bz = cur.bz, w = cur.w;
-      const double gamma = sqrt(1.0 + ux * ux + uy * uy + uz * uz);
-      const double btx = ux / gamma, bty = uy / gamma, btz = uz / gamma;
-      const double bde = btx * ex + bty * ey + btz * ez;
-      const double vx = ex + (bty * bz - btz * by);
-      const double vy = ey + (btz * bx - btx * bz);
-      const double vz = ez + (btx * by - bty * bx);
-      const double rootQ = sqrt(vx * vx + vy * vy + vz * vz - bde * bde);
-      const double chi = rootQ / P.B0;
-      gw = chi * A;
-      const double gchi = w * (A - C);
-      const double ggamma = -2.0 * w * chi * C / gamma;
-      const double coef = gchi / (static_cast<double>(P.B0) * rootQ);
-      gbx = coef * (vy * btz - vz * bty);
-      gby = coef * (vz * btx - vx * btz);
-      gbz = coef * (vx * bty - vy * btx);
-      gex = coef * (vx - bde * btx);
-      gey = coef * (vy - bde * bty);
-      gez = coef * (vz - bde * btz);
-      const double gbtx = coef * ((by * vz - bz * vy) - bde * ex);
-      const double gbty = coef * ((bz * vx - bx * vz) - bde * ey);
-      const double gbtz = coef * ((bx * vy - by * vx) - bde * ez);
-      const double udg = ux * gbtx + uy * gbty + uz * gbtz;
-      const double g3 = gamma * gamma * gamma;
-      gux = gbtx / gamma - ux * udg / g3 + ggamma * ux / gamma;
-      guy = gbty / gamma - uy * udg / g3 + ggamma * uy / gamma;
-      guz = gbtz / gamma - uz * udg / g3 + ggamma * uz / gamma;
+      // sum_e g'_e F and sum_e g'_e dF/dln x at this particle
+      g = emission_grads(o, cur, u.x - u.y * phi, dlnx * u.y, P.em);
     }
+    const double gw = g.gw, gux = g.gu[0], guy = g.gu[1], guz = g.gu[2];
+    const double gbx = g.gb[0], gby = g.gb[1], gbz = g.gb[2];
+    const double gex = g.ge[0], gey = g.ge[1], gez = g.ge[2];
     if (G.gw) G.gw[p] = static_cast<float>(gw);
     if (G.gu[0]) G.gu[0][p] = static_cast<float>(gux);
     if (G.gu[1]) G.gu[1][p] = static_cast<float>(guy);
@@ -449,74 +633,69 @@ __global__ void __launch_bounds__(kThreads) binned_adj_gather(ParticlesDev D, si
   }
 }
 
-// Scratch slots 6..: records, counts, sorted sources, partials, tables.
-struct BinnedScratch {
-  float* recW;
-  float* recPhi;
-  int* recK;
-  unsigned* counts;
-  unsigned* totals;
-  unsigned* bstart;
-  unsigned* istart;
-  float* sortW;
-  float* sortPhi;
-  double* part;
-  double* P0;
-  double* P1;
-};
-
 }  // namespace
 
+// LDS bytes of the bucket pass for a plan (hist, offsets, scan partials and
+// the block's sorted sources).
+static size_t bucket_smem(const SyncPlan& plan) {
+  return sizeof(unsigned) * (2 * static_cast<size_t>(plan.NK) + kThreads) + 2 * sizeof(float) * kChunk;
+}
+
+static size_t subhist_smem(const SyncPlan& plan) {
+  const size_t M1 = plan.M + 1;
+  return sizeof(double) * (2 * M1 + plan.M) + sizeof(unsigned) * (2 * kWin + 1) +
+         sizeof(unsigned short) * plan.JQ;
+}
+
 void sync_forward_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, float* out, hipStream_t st) {
+  RGN_REQUIRE(plan.binned_ok && n < (size_t{1} << 31), RGN_ERR_UNSUPPORTED, "plan outside the binned limits");
   const BinnedDev P = binned_view(plan);
   if (plan.NK == 0 || n == 0) {
     RGN_HIP(hipMemsetAsync(out, 0, sizeof(float) * plan.nbins, st));
     return;
   }
-  const int G = static_cast<int>(std::max<size_t>(1, std::min<size_t>(ceil_div(n, kThreads),
-                                                                         static_cast<size_t>(num_cus()) * 8)));
-  const size_t Q = ceil_div(n, G);
+  const int G = static_cast<int>(ceil_div(n, kChunk));
+  const size_t NK = plan.NK;
   const size_t M1 = plan.M + 1;
-  const size_t max_items = n / kItem + plan.NK + 1;
-  BinnedScratch S{};
-  S.recW = static_cast<float*>(scratch(n * sizeof(float), 6));
-  S.recPhi = static_cast<float*>(scratch(n * sizeof(float), 7));
-  S.recK = static_cast<int*>(scratch(n * sizeof(int), 8));
-  S.counts = static_cast<unsigned*>(scratch(sizeof(unsigned) * plan.NK * static_cast<size_t>(G), 9));
-  unsigned* small = static_cast<unsigned*>(scratch(sizeof(unsigned) * (3 * (plan.NK + 1) + 64), 10));
-  S.totals = small;
-  S.bstart = small + (plan.NK + 1);
-  S.istart = small + 2 * (plan.NK + 1);
-  S.sortW = static_cast<float*>(scratch(n * sizeof(float), 11));
-  S.sortPhi = static_cast<float*>(scratch(n * sizeof(float), 12));
-  S.part = static_cast<double*>(scratch(sizeof(double) * 2 * M1 * max_items, 13));
-  double* pref = static_cast<double*>(scratch(sizeof(double) * 2 * M1 * plan.NK, 14));
-  S.P0 = pref;
-  S.P1 = pref + M1 * plan.NK;
+  // items: sum over k of ceil(totals_k / kItem) <= n / kItem + NK
+  const size_t max_items = n / kItem + NK + 1;
+  float* sortW = static_cast<float*>(scratch(sizeof(float) * G * size_t{kChunk}, 6));
+  float* sortPhi = static_cast<float*>(scratch(sizeof(float) * G * size_t{kChunk}, 7));
+  unsigned* cnt = static_cast<unsigned*>(scratch(sizeof(unsigned) * NK * G, 8));
+  unsigned* loff = static_cast<unsigned*>(scratch(sizeof(unsigned) * NK * G, 9));
+  unsigned* rowoff = static_cast<unsigned*>(scratch(sizeof(unsigned) * NK * (G + 1), 10));
+  unsigned* small = static_cast<unsigned*>(scratch(sizeof(unsigned) * (2 * (NK + 1) + 64), 11));
+  unsigned* totals = small;
+  unsigned* istart = small + (NK + 1);
+  double* partial = static_cast<double*>(scratch(sizeof(double) * 2 * M1 * max_items, 12));
+  double* pref = static_cast<double*>(scratch(sizeof(double) * 2 * M1 * NK, 13));
+  double* P0 = pref;
+  double* P1 = pref + M1 * NK;
 
-  const size_t cnt_smem = sizeof(unsigned) * plan.NK;
-  RGN_REQUIRE(cnt_smem <= 64 * 1024, RGN_ERR_RANGE, "photon bins span too many decades for the binned form");
-  binned_count<<<G, kThreads, cnt_smem, st>>>(D, n, Q, P, S.recW, S.recPhi, S.recK, S.counts);
+  binned_bucket<<<G, kThreads, bucket_smem(plan), st>>>(D, static_cast<unsigned>(n), P, sortW, sortPhi, cnt, loff);
   RGN_HIP(hipGetLastError());
-  binned_scan_rows<<<plan.NK, kThreads, 0, st>>>(S.counts, G, S.totals);
+  stage_mark(st, "binned_bucket");
+  binned_scan_rows<<<plan.NK, kThreads, 0, st>>>(cnt, G, rowoff, totals);
   RGN_HIP(hipGetLastError());
-  binned_scan_totals<<<1, kThreads, 0, st>>>(S.totals, plan.NK, S.bstart, S.istart);
+  stage_mark(st, "binned_scan_rows");
+  binned_scan_items<<<1, kThreads, 0, st>>>(totals, plan.NK, istart);
   RGN_HIP(hipGetLastError());
-  binned_scatter<<<G, kThreads, cnt_smem, st>>>(n, Q, plan.NK, S.recW, S.recPhi, S.recK, S.counts, S.bstart,
-                                               S.sortW, S.sortPhi);
+  stage_mark(st, "binned_scan_items");
+  binned_subhist<<<static_cast<unsigned>(max_items), kSubThreads, subhist_smem(plan), st>>>(
+    P, G, istart, totals, rowoff, loff, sortW, sortPhi, partial);
   RGN_HIP(hipGetLastError());
-  const size_t sub_smem = sizeof(double) * (2 * M1 + plan.M);
-  binned_subhist<<<static_cast<unsigned>(max_items), kThreads, sub_smem, st>>>(P, S.bstart, S.istart, S.sortW,
-                                                                             S.sortPhi, S.part);
+  stage_mark(st, "binned_subhist");
+  binned_prefix<<<plan.NK, kPrefThreads, sizeof(double) * 2 * M1, st>>>(plan.M, istart, partial, P0, P1);
   RGN_HIP(hipGetLastError());
-  binned_prefix<<<plan.NK, kThreads, sizeof(double) * 2 * M1, st>>>(plan.M, S.istart, S.part, S.P0, S.P1);
+  stage_mark(st, "binned_prefix");
+  binned_evaluate<<<plan.nbins, kThreads, 0, st>>>(P, P0, P1, out);
   RGN_HIP(hipGetLastError());
-  binned_evaluate<<<plan.nbins, kThreads, 0, st>>>(P, S.P0, S.P1, out);
-  RGN_HIP(hipGetLastError());
+  stage_mark(st, "binned_evaluate");
 }
 
 void sync_adjoint_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, const float* gbar, float* gw,
                          float* const gu[3], float* const gb[3], float* const ge[3], hipStream_t st) {
+  RGN_REQUIRE(plan.binned_ok && n < (size_t{1} << 31), RGN_ERR_UNSUPPORTED, "plan outside the binned limits");
   const BinnedDev P = binned_view(plan);
   GradPtrs G{};
   G.gw = gw;
@@ -529,16 +708,18 @@ void sync_adjoint_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, 
   const size_t M1 = plan.M + 1;
   double2* U = static_cast<double2*>(scratch(sizeof(double2) * M1 * std::max(plan.NK, 1), 15));
   if (plan.NK > 0) {
-    const size_t tab_smem = plan.nbins * (sizeof(BinInfo) + sizeof(double));
-    binned_adj_table<<<plan.NK, kThreads, tab_smem, st>>>(P, gbar, U);
+    binned_adj_table<<<plan.NK, kThreads, 2 * sizeof(double) * std::max(plan.M, 1), st>>>(P, gbar, U);
     RGN_HIP(hipGetLastError());
+    stage_mark(st, "binned_adj_table");
   }
   BinnedDev Pg = P;
   if (plan.NK == 0) Pg.NK = 0;  // every particle falls outside: gradients are zero
   const int grid = static_cast<int>(std::max<size_t>(1, std::min<size_t>(ceil_div(n, kThreads),
                                                                             static_cast<size_t>(num_cus()) * 8)));
-  binned_adj_gather<<<grid, kThreads, sizeof(double) * std::max(plan.M, 1), st>>>(D, n, Pg, U, G);
+  const size_t smem = sizeof(double) * std::max(plan.M, 1) + sizeof(unsigned short) * plan.JQ;
+  binned_adj_gather<<<grid, kThreads, smem, st>>>(D, n, Pg, U, G);
   RGN_HIP(hipGetLastError());
+  stage_mark(st, "binned_adj_gather");
 }
 
 }  // namespace rgn

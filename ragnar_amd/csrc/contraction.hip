@@ -30,7 +30,7 @@ __global__ void __launch_bounds__(256) kernel_matrix(PlanDev P, const float* __r
   const int g = blockIdx.y;
   if (e >= P.nbins) return;
   const float gb = gbeta[g];
-  const float eps = P.e_at * gb * gb / (P.g_syn * P.g_syn);  // synchrotron.hpp:78-79
+  const float eps = P.em.e_at * gb * gb / (P.em.g_syn * P.em.g_syn);  // synchrotron.hpp:78-79
   const Source s = make_source(dist_log ? gb : 1.0f, eps, P);
   float v = 0.0f;
   if (s.W != 0.0f) {

@@ -110,7 +110,7 @@ struct DistSourceFn {
   };
   __device__ Raw load(size_t g) const { return Raw{gbeta[g], f[g]}; }
   __device__ Source make(const Raw& r, const PlanDev& P) const {
-    const float eps = P.e_at * r.gb * r.gb / (P.g_syn * P.g_syn);
+    const float eps = P.em.e_at * r.gb * r.gb / (P.em.g_syn * P.em.g_syn);
     const float W = dist_log ? r.f * r.gb : r.f;
     return make_source(W, eps, P);
   }
@@ -277,46 +277,14 @@ __global__ void __launch_bounds__(kThreads) adjoint_kernel(ParticlesDev D, size_
     __syncthreads();
     if (p >= n) continue;
     const float2 mine = ac[threadIdx.x];
-    const bool live = s.W != 0.0f;
-    const double ux = cur.ux, uy = cur.uy, uz = cur.uz;
-    const double ex = cur.ex, ey = cur.ey, ez = cur.ez;
-    const double bx = cur.bx, by = cur.by, bz = cur.bz;
-    const double w = cur.w;
-    double gw = 0, gux = 0, guy = 0, guz = 0, gbx = 0, gby = 0, gbz = 0, gex = 0, gey = 0,
-           gez = 0;
-    if (live) {
-      const double A = mine.x;
-      const double C = dlnx * mine.y;
-      const double gamma = sqrt(1.0 + ux * ux + uy * uy + uz * uz);
-      const double btx = ux / gamma, bty = uy / gamma, btz = uz / gamma;
-      const double bde = btx * ex + bty * ey + btz * ez;
-      const double vx = ex + (bty * bz - btz * by);
-      const double vy = ey + (btz * bx - btx * bz);
-      const double vz = ez + (btx * by - bty * bx);
-      const double rootQ = sqrt(vx * vx + vy * vy + vz * vz - bde * bde);
-      const double chi = rootQ / P.B0;
-      gw = chi * A;
-      const double gchi = w * (A - C);
-      const double ggamma = -2.0 * w * chi * C / gamma;
-      const double coef = gchi / (static_cast<double>(P.B0) * rootQ);
-      // dchi/dB = (V x beta) / (B0 sqrt Q)
-      gbx = coef * (vy * btz - vz * bty);
-      gby = coef * (vz * btx - vx * btz);
-      gbz = coef * (vx * bty - vy * btx);
-      // dchi/dE = (V - (beta.E) beta) / (B0 sqrt Q)
-      gex = coef * (vx - bde * btx);
-      gey = coef * (vy - bde * bty);
-      gez = coef * (vz - bde * btz);
-      // dchi/dbeta = (B x V - (beta.E) E) / (B0 sqrt Q)
-      const double gbtx = coef * ((by * vz - bz * vy) - bde * ex);
-      const double gbty = coef * ((bz * vx - bx * vz) - bde * ey);
-      const double gbtz = coef * ((bx * vy - by * vx) - bde * ez);
-      const double udg = ux * gbtx + uy * gbty + uz * gbtz;
-      const double g3 = gamma * gamma * gamma;
-      gux = gbtx / gamma - ux * udg / g3 + ggamma * ux / gamma;
-      guy = gbty / gamma - uy * udg / g3 + ggamma * uy / gamma;
-      guz = gbtz / gamma - uz * udg / g3 + ggamma * uz / gamma;
+    ParticleGrads g{};
+    if (s.W != 0.0f) {
+      const OmegaTerms o = omega_terms(cur, P.em);
+      g = emission_grads(o, cur, mine.x, dlnx * mine.y, P.em);
     }
+    const double gw = g.gw, gux = g.gu[0], guy = g.gu[1], guz = g.gu[2];
+    const double gbx = g.gb[0], gby = g.gb[1], gbz = g.gb[2];
+    const double gex = g.ge[0], gey = g.ge[1], gez = g.ge[2];
     if (G.gw) G.gw[p] = static_cast<float>(gw);
     if (G.gu[0]) G.gu[0][p] = static_cast<float>(gux);
     if (G.gu[1]) G.gu[1][p] = static_cast<float>(guy);
@@ -335,13 +303,11 @@ __global__ void __launch_bounds__(kThreads) particle_terms_kernel(ParticlesDev D
                                                                   PlanDev P, float* __restrict__ out) {
   const size_t p = blockIdx.x * static_cast<size_t>(kThreads) + threadIdx.x;
   if (p >= n) return;
-  float e_peak, chiR;
-  omega_sync(D.u[0][p], D.u[1][p], D.u[2][p], load_or_zero(D.e[0], p), load_or_zero(D.e[1], p),
-             load_or_zero(D.e[2], p), D.b[0][p], D.b[1][p], D.b[2][p], P.B0, P.g_syn, P.e_at,
-             e_peak, chiR);
-  const Source s = particle_source(D, p, P);
-  out[5 * p + 0] = e_peak;
-  out[5 * p + 1] = chiR;
+  const RawParticle r = load_particle(D, p);
+  const OmegaTerms o = omega_terms(r, P.em);
+  const Source s = particle_source(r, P);
+  out[5 * p + 0] = o.e_peak;
+  out[5 * p + 1] = o.chiR;
   out[5 * p + 2] = s.W;
   out[5 * p + 3] = s.vhi;
   out[5 * p + 4] = s.vlo;
@@ -477,9 +443,7 @@ PlanDev device_view(const SyncPlan& plan) {
   P.nbins = plan.nbins;
   P.nbins_pad = plan.nbins_pad;
   P.Z = plan.Z;
-  P.B0 = plan.B0;
-  P.g_syn = plan.g_syn;
-  P.e_at = plan.e_at;
+  P.em = emit_const(plan);
   P.K = plan.K;
   P.win_lo = plan.win_lo;
   P.win_hi = plan.win_hi;
@@ -522,6 +486,7 @@ void launch_forward(const SyncPlan& plan, SourceFn fn, size_t n, int grid, doubl
       throw Error(RGN_ERR_UNSUPPORTED, "at most 512 photon bins per plan");
   }
   RGN_HIP(hipGetLastError());
+  stage_mark(st, "forward_kernel");
 }
 
 void launch_combine(const SyncPlan& plan, const double* partials, int grid, float* out,
@@ -529,21 +494,28 @@ void launch_combine(const SyncPlan& plan, const double* partials, int grid, floa
   combine_kernel<<<plan.nbins, kThreads, 0, st>>>(partials, grid, plan.bine.as<float>(),
                                                   plan.nbins, out);
   RGN_HIP(hipGetLastError());
+  stage_mark(st, "combine_kernel");
 }
 
 // Direct pair-by-pair kernels below ~2^28 evals (the factorised form has a
 // fixed cost of seven launches); the factorised form above.  RAGNAR_SYNC_METHOD
 // (direct | binned) overrides the automatic choice.
 int resolve_method(const SyncPlan& plan, size_t n, int method) {
+  // particle indices of the factorised kernels are 32-bit
+  const bool binned_fits = plan.binned_ok && n < (size_t{1} << 31);
   if (method == RGN_METHOD_AUTO) {
     if (const char* m = std::getenv("RAGNAR_SYNC_METHOD")) {
       if (std::string(m) == "direct") return RGN_METHOD_DIRECT;
-      if (std::string(m) == "binned") return RGN_METHOD_BINNED;
+      if (std::string(m) == "binned" && binned_fits) return RGN_METHOD_BINNED;
     }
-    return static_cast<double>(n) * plan.nbins >= 268435456.0 ? RGN_METHOD_BINNED : RGN_METHOD_DIRECT;
+    return binned_fits && static_cast<double>(n) * plan.nbins >= 268435456.0 ? RGN_METHOD_BINNED
+                                                                             : RGN_METHOD_DIRECT;
   }
   RGN_REQUIRE(method == RGN_METHOD_DIRECT || method == RGN_METHOD_BINNED, RGN_ERR_INVALID,
               "unknown method");
+  RGN_REQUIRE(method != RGN_METHOD_BINNED || binned_fits, RGN_ERR_UNSUPPORTED,
+              "the binned method needs fewer than 2^31 particles, at most 4096 distinct bin "
+              "offsets and a bin range of at most 4096 table units");
   return method;
 }
 
@@ -654,6 +626,42 @@ int rgn_sync_adjoint_method(const rgn_sync_plan* plan, const float* const d_u[3]
       adjoint_kernel<32, 16><<<grid, kThreads, smem, st>>>(D, n, P, d_gbar, G);
     }
     RGN_HIP(hipGetLastError());
+    stage_mark(st, "adjoint_kernel");
+  });
+}
+
+namespace {
+// Rethrows the status of a nested C-ABI call.
+void rethrow(int status) {
+  if (status != RGN_OK) throw Error(status, rgn_last_error());
+}
+}  // namespace
+
+int rgn_sync_spectrum_stages(const rgn_sync_plan* plan, const float* const d_u[3],
+                             const float* const d_e[3], const float* const d_b[3],
+                             const float* d_w, size_t n, int method, float* d_out,
+                             rgn_stream_t stream, float* stage_ms, const char** names,
+                             int max_stages, int* nstages) {
+  return guarded([&] {
+    RGN_REQUIRE(stage_ms != nullptr && nstages != nullptr, RGN_ERR_INVALID, "null timing output");
+    StageScope scope(as_stream(stream));
+    rethrow(rgn_sync_spectrum_method(plan, d_u, d_e, d_b, d_w, n, method, d_out, stream));
+    *nstages = scope.finish(stage_ms, names, max_stages);
+  });
+}
+
+int rgn_sync_adjoint_stages(const rgn_sync_plan* plan, const float* const d_u[3],
+                            const float* const d_e[3], const float* const d_b[3],
+                            const float* d_w, size_t n, const float* d_gbar, int method,
+                            float* d_gw, float* const d_gu[3], float* const d_gb[3],
+                            float* const d_ge[3], rgn_stream_t stream, float* stage_ms,
+                            const char** names, int max_stages, int* nstages) {
+  return guarded([&] {
+    RGN_REQUIRE(stage_ms != nullptr && nstages != nullptr, RGN_ERR_INVALID, "null timing output");
+    StageScope scope(as_stream(stream));
+    rethrow(rgn_sync_adjoint_method(plan, d_u, d_e, d_b, d_w, n, d_gbar, method, d_gw, d_gu, d_gb,
+                                    d_ge, stream));
+    *nstages = scope.finish(stage_ms, names, max_stages);
   });
 }
 
@@ -690,13 +698,16 @@ int rgn_sync_particle_terms(const rgn_sync_plan* plan, const float* const d_u[3]
 }
 
 // Plan scalars for tests: Z, TP, NB, K, Amin, Amax.
-int rgn_sync_plan_info(const rgn_sync_plan* plan, int* ints3, double* dbl3) {
+int rgn_sync_plan_info(const rgn_sync_plan* plan, int* ints6, double* dbl3) {
   return guarded([&] {
     RGN_REQUIRE(plan != nullptr, RGN_ERR_INVALID, "null plan");
-    if (ints3) {
-      ints3[0] = plan->p.Z;
-      ints3[1] = plan->p.TP;
-      ints3[2] = plan->p.NB;
+    if (ints6) {
+      ints6[0] = plan->p.Z;
+      ints6[1] = plan->p.TP;
+      ints6[2] = plan->p.NB;
+      ints6[3] = plan->p.M;
+      ints6[4] = plan->p.NK;
+      ints6[5] = plan->p.binned_ok ? 1 : 0;
     }
     if (dbl3) {
       dbl3[0] = plan->p.K;

@@ -1,6 +1,7 @@
 // Host-side plan of the emission kernels (shared by emission.hip and cells.hip).
 #pragma once
 
+#include <cmath>
 #include <vector>
 
 #include "emission_common.hpp"
@@ -32,12 +33,39 @@ struct SyncPlan {
   int T = 0;                       // table nodes
   int kmin = 0, NK = 0;            // integer parts of B that can contribute
   int M = 0;                       // distinct alpha_e
+  int JQ = 0;                      // cells of the sub-cell lookup grid on [0, 1)
+  bool binned_ok = false;          // the plan fits the factorised kernels' limits
   std::vector<double> A;           // A_e of live bins
   std::vector<char> live;          // bin e has e_e > 0
   DeviceBuffer alpha;              // double[M], sorted
+  DeviceBuffer jlut;               // uint16[JQ]: number of alphas below q / JQ
+  DeviceBuffer rptr;               // int[M + 1]: live bins with alpha rank r are
+  DeviceBuffer rlist;              //   rlist[rptr[r] .. rptr[r + 1]) (ascending e)
   DeviceBuffer binfo;              // BinInfo[nbins]
   DeviceBuffer seg;                // SegInfo[T - 1]
 };
+
+// The plan's emission constants with the correctly rounded reciprocals of the
+// fixed divisors B0 and float(g_syn * g_syn) (host division: RN(1 / d)).
+inline EmitConst emit_const(const SyncPlan& p) {
+  EmitConst c{};
+  c.B0 = p.B0;
+  c.g_syn = p.g_syn;
+  c.e_at = p.e_at;
+  const float g2 = p.g_syn * p.g_syn;
+  c.Dg = static_cast<double>(g2);
+  c.rB0 = 1.0 / static_cast<double>(p.B0);
+  c.rDg = 1.0 / c.Dg;
+  c.fast = std::isnormal(static_cast<double>(p.B0)) && std::isnormal(c.Dg) && std::isnormal(c.rB0) &&
+           std::isnormal(c.rDg);
+  return c;
+}
+
+// Limits of the factorised kernels (LDS-resident per-bucket and per-sub-cell
+// arrays, each kernel within 64 KB of LDS); plans beyond them use the direct
+// kernels.
+constexpr int kBinnedMaxM = 1024;
+constexpr int kBinnedMaxNK = 3072;
 
 // Per photon bin of the factorised form.
 struct BinInfo {

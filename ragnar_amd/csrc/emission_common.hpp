@@ -37,6 +37,16 @@ struct ParticlesDev {
   const float* w;
 };
 
+// Constants of the emission formula with the correctly rounded reciprocals of
+// its fixed divisors (see omega_terms).
+struct EmitConst {
+  float B0, g_syn, e_at;
+  double rB0;  // RN(1 / B0)
+  double Dg;   // the reference's e_peak divisor: float(g_syn * g_syn) as double
+  double rDg;  // RN(1 / Dg)
+  int fast;    // B0 and Dg are normal and their reciprocals finite
+};
+
 struct PlanDev {
   const float2* table;  // (c_j, s_j), TP entries
   const float2* bincs;  // (c_e, S_e), nbins_pad entries
@@ -45,7 +55,7 @@ struct PlanDev {
   int nbins;
   int nbins_pad;
   int Z;
-  float B0, g_syn, e_at;
+  EmitConst em;
   double K;         // table units per unit of log2 x
   double win_lo;    // a particle contributes iff win_lo < B_p < win_hi
   double win_hi;
@@ -62,26 +72,103 @@ __device__ __forceinline__ unsigned cvt_sat_u32(float u) {
   return r;
 }
 
+// The raw inputs of one particle (loaded ahead of their use).
+struct RawParticle {
+  float ux, uy, uz, ex, ey, ez, bx, by, bz, w;
+};
+
+// Correctly rounded a / b from y = RN(1 / b).  Markstein's theorem (radix 2,
+// round to nearest, no underflow or overflow): if y is within half an ulp of
+// 1/b and q0 within one ulp of a/b, the residual a - b q0 is exact in one FMA
+// and q0 + (a - b q0) y rounds to RN(a/b).  q0 = RN(a y) is within one ulp.
+// Three dependent FMA-class operations instead of a full division.
+__device__ __forceinline__ double div_rn(double a, double b, double y) {
+  const double q0 = a * y;
+  return fma(fma(-q0, b, a), y, q0);
+}
+
+// Intermediate terms of OmegaSync_ChiR for one particle (the adjoint reuses them).
+struct OmegaTerms {
+  double gamma, ig;        // gamma and RN(1 / gamma)
+  double btx, bty, btz;    // beta = u / gamma
+  double bde;              // beta . E
+  double vx, vy, vz;       // V = E + beta x B
+  double rootQ;            // sqrt(|V|^2 - (beta . E)^2)
+  float e_peak, chiR;
+};
+
 // synchrotron.hpp:181-232 (Kernel<D>::OmegaSync_ChiR), evaluated with the
 // reference's promotions: squares of the float momenta are float products,
 // everything from gamma on is double, chiR is rounded to float and re-read as
 // float in e_peak.  The library is compiled with -ffp-contract=off, so these
-// expressions round exactly as written.
-__device__ __forceinline__ void omega_sync(float ux, float uy, float uz, float ex, float ey,
-                                           float ez, float bx, float by, float bz, float B0,
-                                           float g_syn, float e_at, float& e_peak,
-                                           float& chiR) {
+// expressions round exactly as written.  The five double divisions (three by
+// gamma, one by B0, one by g_syn^2) are correctly rounded through div_rn, which
+// returns the same doubles as the division operator; a particle with a
+// non-finite gamma, or a plan whose divisors are not normal, takes the
+// division operator itself.
+__device__ __forceinline__ OmegaTerms omega_terms(const RawParticle& r, const EmitConst& C) {
+  OmegaTerms o;
+  const float ux = r.ux, uy = r.uy, uz = r.uz, ex = r.ex, ey = r.ey, ez = r.ez;
+  const float bx = r.bx, by = r.by, bz = r.bz;
   const double gamma = sqrt(1.0 + ux * ux + uy * uy + uz * uz);
-  const double beta_x = ux / gamma;
-  const double beta_y = uy / gamma;
-  const double beta_z = uz / gamma;
-  const double beta_dot_e = beta_x * ex + beta_y * ey + beta_z * ez;
-  const double vx = ex + (beta_y * bz - beta_z * by);
-  const double vy = ey + (beta_z * bx - beta_x * bz);
-  const double vz = ez + (beta_x * by - beta_y * bx);
-  const double v_sqr = vx * vx + vy * vy + vz * vz;
-  chiR = static_cast<float>(sqrt(v_sqr - beta_dot_e * beta_dot_e) / B0);
-  e_peak = static_cast<float>(e_at * gamma * gamma * chiR / (g_syn * g_syn));
+  o.gamma = gamma;
+  const bool fast = C.fast && gamma < 1e300;  // false for inf and NaN
+  if (fast) {
+    const double ig = 1.0 / gamma;
+    o.ig = ig;
+    o.btx = div_rn(ux, gamma, ig);
+    o.bty = div_rn(uy, gamma, ig);
+    o.btz = div_rn(uz, gamma, ig);
+  } else {
+    o.ig = 1.0 / gamma;
+    o.btx = ux / gamma;
+    o.bty = uy / gamma;
+    o.btz = uz / gamma;
+  }
+  o.bde = o.btx * ex + o.bty * ey + o.btz * ez;
+  o.vx = ex + (o.bty * bz - o.btz * by);
+  o.vy = ey + (o.btz * bx - o.btx * bz);
+  o.vz = ez + (o.btx * by - o.bty * bx);
+  const double v_sqr = o.vx * o.vx + o.vy * o.vy + o.vz * o.vz;
+  o.rootQ = sqrt(v_sqr - o.bde * o.bde);
+  if (fast) {
+    o.chiR = static_cast<float>(div_rn(o.rootQ, static_cast<double>(C.B0), C.rB0));
+    o.e_peak = static_cast<float>(div_rn(C.e_at * gamma * gamma * o.chiR, C.Dg, C.rDg));
+  } else {
+    o.chiR = static_cast<float>(o.rootQ / C.B0);
+    o.e_peak = static_cast<float>(C.e_at * gamma * gamma * o.chiR / (C.g_syn * C.g_syn));
+  }
+  return o;
+}
+
+// log2 of a positive finite float, accurate to ~1e-15 absolute:
+// x = m 2^E with m in [sqrt(1/2), sqrt(2)), log2 m = (2 / ln 2) atanh(z),
+// z = (m - 1) / (m + 1), |z| <= 0.1716, odd series through z^15 (truncation
+// below 2e-14).  z is formed with a refined reciprocal (relative error ~1e-16).
+__device__ __forceinline__ double log2_pos(float xf) {
+  int E;
+  double m = frexp(static_cast<double>(xf), &E);  // [0.5, 1)
+  if (m < 0.70710678118654752440) {
+    m *= 2.0;
+    E -= 1;
+  }
+  const double num = m - 1.0, den = m + 1.0;  // both exact
+  double y = __builtin_amdgcn_rcp(den);
+  y = fma(y, fma(-den, y, 1.0), y);
+  y = fma(y, fma(-den, y, 1.0), y);
+  double z = num * y;
+  z = fma(fma(-z, den, num), y, z);
+  const double s = z * z;
+  constexpr double c1 = 2.0 / 0.69314718055994530942;
+  double p = c1 / 15.0;
+  p = fma(p, s, c1 / 13.0);
+  p = fma(p, s, c1 / 11.0);
+  p = fma(p, s, c1 / 9.0);
+  p = fma(p, s, c1 / 7.0);
+  p = fma(p, s, c1 / 5.0);
+  p = fma(p, s, c1 / 3.0);
+  p = fma(p, s, c1);
+  return static_cast<double>(E) + z * p;
 }
 
 // Per-source interpolation parameters: weight W (multiplies F) and B split as
@@ -93,8 +180,8 @@ struct Source {
 
 __device__ __forceinline__ Source make_source(float W, float eps, const PlanDev& P) {
   Source s{0.0f, P.v_safe, 0.0f};
-  if (eps > 0.0f) {  // synchrotron.hpp:162 (`e_peak > 0.0`); NaN fails too
-    const double B = log2(static_cast<double>(eps)) * P.K;
+  if (eps > 0.0f && eps < INFINITY) {  // synchrotron.hpp:162 (`e_peak > 0.0`); NaN fails too
+    const double B = log2_pos(eps) * P.K;
     if (B > P.win_lo && B < P.win_hi) {
       const double hi = rint(B);
       s.W = W;
@@ -108,11 +195,6 @@ __device__ __forceinline__ Source make_source(float W, float eps, const PlanDev&
 __device__ __forceinline__ float load_or_zero(const float* p, size_t i) {
   return p ? p[i] : 0.0f;
 }
-
-// The raw inputs of one particle (loaded ahead of their use).
-struct RawParticle {
-  float ux, uy, uz, ex, ey, ez, bx, by, bz, w;
-};
 
 __device__ __forceinline__ RawParticle load_particle(const ParticlesDev& D, size_t p) {
   RawParticle r;
@@ -130,16 +212,65 @@ __device__ __forceinline__ RawParticle load_particle(const ParticlesDev& D, size
 }
 
 __device__ __forceinline__ Source particle_source(const RawParticle& r, const PlanDev& P) {
-  float e_peak, chiR;
-  omega_sync(r.ux, r.uy, r.uz, r.ex, r.ey, r.ez, r.bx, r.by, r.bz, P.B0, P.g_syn, P.e_at, e_peak,
-             chiR);
-  return make_source(chiR * r.w, e_peak, P);
+  const OmegaTerms o = omega_terms(r, P.em);
+  return make_source(o.chiR * r.w, o.e_peak, P);
 }
 
 // Source of particle p of a particle store.
 __device__ __forceinline__ Source particle_source(const ParticlesDev& D, size_t p,
                                                   const PlanDev& P) {
   return particle_source(load_particle(D, p), P);
+}
+
+// Gradients of L = sum_e gbar_e S_e with respect to one particle's w, u, B, E,
+// given A = sum_e g'_e F_e and Cl = sum_e g'_e dF_e/dln x (g'_e = gbar_e e_e)
+// at that particle.  With chi = rootQ / B0, W = chi w and x_e = e_e / eps,
+// eps = e_at gamma^2 chi / g_syn^2:
+//   dL/dw = chi A,  dL/dchi = w (A - Cl),  dL/dgamma = -2 w chi Cl / gamma,
+//   dchi/dV = V / (B0 rootQ), dchi/d(beta.E) = -(beta.E) / (B0 rootQ),
+//   dbeta_i/du_j = delta_ij / gamma - u_i u_j / gamma^3, dgamma/du_j = u_j / gamma.
+// Gradients need no correct rounding: the divisions are reciprocal products.
+struct ParticleGrads {
+  double gw, gu[3], gb[3], ge[3];
+};
+
+// 1 / x to about one ulp: hardware estimate and two Newton steps.
+__device__ __forceinline__ double rcp_nr(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  y = fma(y, fma(-x, y, 1.0), y);
+  return fma(y, fma(-x, y, 1.0), y);
+}
+
+__device__ __forceinline__ ParticleGrads emission_grads(const OmegaTerms& o, const RawParticle& r,
+                                                        double A, double Cl, const EmitConst& C) {
+  ParticleGrads g;
+  const double ig = o.ig;
+  const double chi = o.rootQ * C.rB0;
+  const double w = r.w;
+  g.gw = chi * A;
+  const double gchi = w * (A - Cl);
+  const double ggamma = -2.0 * w * chi * Cl * ig;
+  const double coef = gchi * C.rB0 * rcp_nr(o.rootQ);
+  const double ex = r.ex, ey = r.ey, ez = r.ez, bx = r.bx, by = r.by, bz = r.bz;
+  // dchi/dB = (V x beta) / (B0 rootQ)
+  g.gb[0] = coef * (o.vy * o.btz - o.vz * o.bty);
+  g.gb[1] = coef * (o.vz * o.btx - o.vx * o.btz);
+  g.gb[2] = coef * (o.vx * o.bty - o.vy * o.btx);
+  // dchi/dE = (V - (beta.E) beta) / (B0 rootQ)
+  g.ge[0] = coef * (o.vx - o.bde * o.btx);
+  g.ge[1] = coef * (o.vy - o.bde * o.bty);
+  g.ge[2] = coef * (o.vz - o.bde * o.btz);
+  // dchi/dbeta = (B x V - (beta.E) E) / (B0 rootQ)
+  const double gbtx = coef * ((by * o.vz - bz * o.vy) - o.bde * ex);
+  const double gbty = coef * ((bz * o.vx - bx * o.vz) - o.bde * ey);
+  const double gbtz = coef * ((bx * o.vy - by * o.vx) - o.bde * ez);
+  const double ux = r.ux, uy = r.uy, uz = r.uz;
+  const double udg = ux * gbtx + uy * gbty + uz * gbtz;
+  const double t = ggamma - udg * ig * ig;
+  g.gu[0] = ig * (gbtx + ux * t);
+  g.gu[1] = ig * (gbty + uy * t);
+  g.gu[2] = ig * (gbtz + uz * t);
+  return g;
 }
 
 // DPP row/bank controls (GFX9 encoding).

@@ -2,6 +2,7 @@
 // the per-device scratch cache.
 #include "rgn_core.hpp"
 
+#include <algorithm>
 #include <cstdlib>
 #include <mutex>
 #include <string>
@@ -77,6 +78,44 @@ void DeviceBuffer::release() {
     ptr = nullptr;
     bytes = 0;
   }
+}
+
+namespace {
+thread_local StageScope* t_stages = nullptr;
+}
+
+StageScope::StageScope(hipStream_t s) : st(s) {
+  RGN_REQUIRE(t_stages == nullptr, RGN_ERR_INVALID, "nested stage timing");
+  for (auto& e : ev) RGN_HIP(hipEventCreate(&e));
+  RGN_HIP(hipEventRecord(ev[0], st));
+  t_stages = this;
+}
+
+StageScope::~StageScope() {
+  t_stages = nullptr;
+  for (auto& e : ev) {
+    if (e) (void)hipEventDestroy(e);
+  }
+}
+
+int StageScope::finish(float* ms, const char** out_names, int max_stages) {
+  t_stages = nullptr;
+  if (n > 0) RGN_HIP(hipEventSynchronize(ev[n]));
+  const int m = std::min(n, max_stages);
+  for (int i = 0; i < m; ++i) {
+    RGN_HIP(hipEventElapsedTime(ms + i, ev[i], ev[i + 1]));
+    if (out_names) out_names[i] = names[i];
+  }
+  return n;
+}
+
+void stage_mark(hipStream_t st, const char* name) {
+  StageScope* s = t_stages;
+  if (!s || s->n >= kMaxStages) return;
+  RGN_REQUIRE(st == s->st, RGN_ERR_INVALID, "stage timing across streams");
+  s->names[s->n] = name;
+  RGN_HIP(hipEventRecord(s->ev[s->n + 1], st));
+  ++s->n;
 }
 
 void* scratch(size_t bytes, int slot) {

@@ -83,6 +83,22 @@ int env_int(const char* name, int def);
 
 inline hipStream_t as_stream(rgn_stream_t s) { return static_cast<hipStream_t>(s); }
 
+// Per-launch timing of one library call (the *_stages diagnostics entry
+// points).  While a StageScope is alive on this thread, stage_mark records a
+// HIP event on the launch stream after each kernel launch; finish() waits for
+// the last event and returns the elapsed milliseconds of every launch.
+constexpr int kMaxStages = 16;
+struct StageScope {
+  explicit StageScope(hipStream_t st);
+  ~StageScope();
+  int finish(float* ms, const char** names, int max_stages);
+  hipStream_t st;
+  hipEvent_t ev[kMaxStages + 1] = {};
+  const char* names[kMaxStages] = {};
+  int n = 0;
+};
+void stage_mark(hipStream_t st, const char* name);
+
 inline size_t ceil_div(size_t a, size_t b) { return (a + b - 1) / b; }
 
 }  // namespace rgn

@@ -16,7 +16,7 @@ from ._capi import FTable, HistPlan, SyncPlan, check, lib, ptr3
 
 __all__ = ["FTable", "SyncPlan", "HistPlan", "sync_spectrum", "sync_adjoint", "sync_cells",
            "sync_fromdist", "sync_fromdist_batched", "energy_hist", "particle_terms",
-           "SynchrotronSpectrumFn", "synchrotron_spectrum"]
+           "spectrum_stages", "adjoint_stages", "SynchrotronSpectrumFn", "synchrotron_spectrum"]
 
 _default_table: FTable | None = None
 
@@ -90,6 +90,49 @@ def sync_adjoint(plan: SyncPlan, u, b, w, e, gbar, need=("w", "u", "b", "e"), me
         _soa(gu, "gu", n) or none3, _soa(gb, "gb", n) or none3, _soa(ge, "ge", n) or none3,
         _stream()))
     return gw, gu, gb, ge
+
+
+def _stage_buffers():
+    return (C.c_float * 16)(), (C.c_char_p * 16)(), C.c_int()
+
+
+def _stage_list(ms, names, k):
+    return [(names[i].decode(), float(ms[i])) for i in range(min(k.value, 16))]
+
+
+def spectrum_stages(plan: SyncPlan, u, b, w=None, e=None, out=None, method="auto"):
+    """One forward call timed per kernel launch with HIP events on the current
+    stream: [(kernel name, ms), ...] in launch order."""
+    n = u.shape[1]
+    if out is None:
+        out = torch.empty(plan.nbins, dtype=torch.float32, device=u.device)
+    ms, names, k = _stage_buffers()
+    check(lib().rgn_sync_spectrum_stages(plan.handle, _soa(u, "u"),
+                                         _soa(e, "e", n) or ptr3(None, None, None),
+                                         _soa(b, "b", n), _vec(w, "w", n), n, METHODS[method],
+                                         out.data_ptr(), _stream(), C.cast(ms, C.c_void_p),
+                                         C.cast(names, C.c_void_p), 16, C.byref(k)))
+    return _stage_list(ms, names, k)
+
+
+def adjoint_stages(plan: SyncPlan, u, b, w, e, gbar, need=("w", "u", "b", "e"), method="auto"):
+    """One adjoint call timed per kernel launch: [(kernel name, ms), ...]."""
+    n = u.shape[1]
+    dev = u.device
+    gw = torch.empty(n, dtype=torch.float32, device=dev) if "w" in need else None
+    gu = torch.empty(3, n, dtype=torch.float32, device=dev) if "u" in need else None
+    gb = torch.empty(3, n, dtype=torch.float32, device=dev) if "b" in need else None
+    ge = torch.empty(3, n, dtype=torch.float32, device=dev) if "e" in need else None
+    gbar = gbar.contiguous().float()
+    none3 = ptr3(None, None, None)
+    ms, names, k = _stage_buffers()
+    check(lib().rgn_sync_adjoint_stages(
+        plan.handle, _soa(u, "u"), _soa(e, "e", n) or none3, _soa(b, "b", n), _vec(w, "w", n), n,
+        _vec(gbar, "gbar", plan.nbins), METHODS[method],
+        gw.data_ptr() if gw is not None else None,
+        _soa(gu, "gu", n) or none3, _soa(gb, "gb", n) or none3, _soa(ge, "ge", n) or none3,
+        _stream(), C.cast(ms, C.c_void_p), C.cast(names, C.c_void_p), 16, C.byref(k)))
+    return _stage_list(ms, names, k)
 
 
 def sync_cells(plan: SyncPlan, u, b, offsets, w=None, e=None, out=None) -> torch.Tensor:

@@ -65,17 +65,29 @@ def test_ffunc_integrand(orc):
         assert np.float32(rg.Ffunc_integrand(x)) == orc.ffunc_integrand(x), x
 
 
-@pytest.mark.parametrize("seed", [1, 2])
-def test_particle_terms_bit_exact(orc, table, seed):
-    u, b = workloads.particles(3000, seed=seed)
+@pytest.mark.parametrize("seed,consts", [(1, (1.3, 2.0, 0.7)), (2, (1.3, 2.0, 0.7)),
+                                         (3, (1.0, 1.0, 1.0)), (4, (0.37, 3.1e-3, 41.0))])
+def test_particle_terms_bit_exact(orc, table, seed, consts):
+    """e_peak and chiR of every particle equal the reference arithmetic bit for
+    bit, over momenta from 1e-30 to 1e18 (including exact zeros and momenta
+    whose squares overflow float), fields of both signs and several
+    (B0, g_syn, e_at)."""
+    n = 4000
     rng = np.random.default_rng(seed)
-    e = rng.normal(0, 0.05, (3, 3000)).astype(np.float32)
+    u, b = workloads.particles(n, seed=seed)
+    scale = 10.0 ** rng.uniform(-30, 18, n)
+    u[:, : n // 2] *= scale[: n // 2].astype(np.float32)
+    u[:, 100:110] = 0.0
+    u[0, 110:120] = -0.0
+    e = rng.normal(0, 0.05, (3, n)).astype(np.float32)
+    e[:, : n // 4] *= -3.0
     bins = orc.logspace(1e-2, 1e7, 256)
-    plan = T.SyncPlan(table, bins, 1.3, 2.0, 0.7)
+    plan = T.SyncPlan(table, bins, *consts)
     terms = T.particle_terms(plan, dev(u), dev(b), None, dev(e)).cpu().numpy()
-    for p in range(0, 3000, 7):
-        ep, chi = orc.omega_sync(u[:, p], e[:, p], b[:, p], 1.3, 2.0, 0.7)
-        assert terms[p, 0] == ep and terms[p, 1] == chi, p
+    for p in range(n):
+        ep, chi = orc.omega_sync(u[:, p], e[:, p], b[:, p], *consts)
+        assert np.array_equal(terms[p, 0], ep, equal_nan=True), (p, terms[p, 0], ep)
+        assert np.array_equal(terms[p, 1], chi, equal_nan=True), (p, terms[p, 1], chi)
 
 
 METHODS = ["direct", "binned"]
