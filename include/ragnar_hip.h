@@ -114,9 +114,12 @@ int rgn_sync_adjoint(const rgn_sync_plan* plan, const float* const d_u[3],
  * factorisation of the piecewise-linear interpolant: particles are partitioned
  * by the integer part of their table coordinate and reduced into per-sub-cell
  * moments (forward), or read one precomputed entry each (adjoint); cost is
- * ~independent of the bin count.  RGN_METHOD_AUTO picks binned from 2^28 pairs
- * (env RAGNAR_SYNC_METHOD=direct|binned overrides).  Same results to
- * FP32/FP64 rounding (no reference counterpart). */
+ * ~independent of the bin count.  RGN_METHOD_AUTO picks the cheaper one from a
+ * measured cost model (binned from ~2^20 particles at 256 bins; env
+ * RAGNAR_SYNC_METHOD=direct|binned overrides).  Binned needs fewer than 2^31
+ * particles, at most 1024 distinct fractional bin offsets and a bin range of
+ * at most 3072 table units (RGN_ERR_UNSUPPORTED otherwise; AUTO then runs
+ * direct).  Same results to FP32/FP64 rounding (no reference counterpart). */
 enum { RGN_METHOD_AUTO = 0, RGN_METHOD_DIRECT = 1, RGN_METHOD_BINNED = 2 };
 int rgn_sync_spectrum_method(const rgn_sync_plan* plan, const float* const d_u[3],
                              const float* const d_e[3], const float* const d_b[3],

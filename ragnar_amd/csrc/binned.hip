@@ -186,12 +186,25 @@ __device__ T block_scan_exclusive(PA a, PO o, int N, T* part) {
   return total;
 }
 
+// Run descriptor of bucket k in bucket block b: start << 16 | length (both at
+// most kChunk = 4096).  The table is zero-filled first and only non-empty runs
+// are written (each scattered 4-byte store costs a 32-byte write-back).
+__device__ __forceinline__ unsigned run_len(unsigned r) { return r & 0xffffu; }
+__device__ __forceinline__ unsigned run_start(unsigned r) { return r >> 16; }
+
+__global__ void __launch_bounds__(kThreads) fill_zero(unsigned* __restrict__ p, size_t n) {
+  for (size_t i = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * kThreads) {
+    p[i] = 0u;
+  }
+}
+
 // Pass 1: per block of kChunk particles, sources sorted by bucket k.
 //   sortW/sortPhi[b * kChunk + i]: the block's sources in bucket order;
-//   cnt[k * G + b], loff[k * G + b]: run length and start of bucket k in block b.
+//   runs[k * G + b]: run descriptor of bucket k in block b.
 __global__ void __launch_bounds__(kThreads) binned_bucket(ParticlesDev D, unsigned n, BinnedDev P,
                                                           float* __restrict__ sortW, float* __restrict__ sortPhi,
-                                                          unsigned* __restrict__ cnt, unsigned* __restrict__ loff) {
+                                                          unsigned* __restrict__ runs) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int NK = P.NK;
   unsigned* hist = reinterpret_cast<unsigned*>(smem);
@@ -230,8 +243,7 @@ __global__ void __launch_bounds__(kThreads) binned_bucket(ParticlesDev D, unsign
   __syncthreads();
   block_scan_exclusive<unsigned>(hist, off, NK, part);
   for (int k = threadIdx.x; k < NK; k += kThreads) {
-    cnt[static_cast<size_t>(k) * G + blockIdx.x] = hist[k];
-    loff[static_cast<size_t>(k) * G + blockIdx.x] = off[k];
+    if (hist[k]) runs[static_cast<size_t>(k) * G + blockIdx.x] = off[k] << 16 | hist[k];
   }
 #pragma unroll
   for (int i = 0; i < kPer; ++i) {
@@ -249,15 +261,22 @@ __global__ void __launch_bounds__(kThreads) binned_bucket(ParticlesDev D, unsign
   }
 }
 
-// Pass 2a: per bucket k, rowoff[k * (G + 1) + b] = sum of cnt[k][b' < b];
-// rowoff[k * (G + 1) + G] = totals[k].
-__global__ void __launch_bounds__(kThreads) binned_scan_rows(const unsigned* __restrict__ cnt, int G,
+// Run lengths of one bucket's row of the run table.
+struct RunLengths {
+  const unsigned* r;
+  __device__ unsigned operator[](int i) const { return run_len(r[i]); }
+};
+
+// Pass 2a: per bucket k, rowoff[k * (G + 1) + b] = sum of the run lengths of
+// blocks b' < b; rowoff[k * (G + 1) + G] = totals[k].
+__global__ void __launch_bounds__(kThreads) binned_scan_rows(const unsigned* __restrict__ runs, int G,
                                                              unsigned* __restrict__ rowoff,
                                                              unsigned* __restrict__ totals) {
   __shared__ unsigned part[kThreads];
   const int k = blockIdx.x;
   unsigned* row = rowoff + static_cast<size_t>(k) * (G + 1);
-  const unsigned total = block_scan_exclusive<unsigned>(cnt + static_cast<size_t>(k) * G, row, G, part);
+  const unsigned total =
+    block_scan_exclusive<unsigned>(RunLengths{runs + static_cast<size_t>(k) * G}, row, G, part);
   if (threadIdx.x == 0) {
     row[G] = total;
     totals[k] = total;
@@ -283,7 +302,7 @@ __global__ void __launch_bounds__(kThreads) binned_scan_items(const unsigned* __
 __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G, const unsigned* __restrict__ istart,
                                                               const unsigned* __restrict__ totals,
                                                               const unsigned* __restrict__ rowoff,
-                                                              const unsigned* __restrict__ loff,
+                                                              const unsigned* __restrict__ runs,
                                                               const float* __restrict__ sortW,
                                                               const float* __restrict__ sortPhi,
                                                               double* __restrict__ partial) {
@@ -320,7 +339,7 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
   const unsigned v0 = (it - istart[k]) * kItem;
   const unsigned v1 = min(totals[k], v0 + kItem);
   const unsigned* row = rowoff + static_cast<size_t>(k) * (G + 1);
-  const unsigned* lrow = loff + static_cast<size_t>(k) * G;
+  const unsigned* lrow = runs + static_cast<size_t>(k) * G;
   // b_lo = #{b < G : row[b] <= v0} - 1, b_hi = #{b < G : row[b] <= v1 - 1} - 1
   {
     unsigned c0 = 0, c1 = 0;
@@ -352,7 +371,7 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
         const unsigned ra = max(r0, v0), rz = min(row[ws + i + 1], v1);
         if (ra < rz) {
           fa[q] = ra;
-          fidx[q] = static_cast<unsigned>(ws + i) * kChunk + lrow[ws + i] + (ra - r0);
+          fidx[q] = static_cast<unsigned>(ws + i) * kChunk + run_start(lrow[ws + i]) + (ra - r0);
           ++nfl;
         }
       }
@@ -661,8 +680,7 @@ void sync_forward_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, 
   const size_t max_items = n / kItem + NK + 1;
   float* sortW = static_cast<float*>(scratch(sizeof(float) * G * size_t{kChunk}, 6));
   float* sortPhi = static_cast<float*>(scratch(sizeof(float) * G * size_t{kChunk}, 7));
-  unsigned* cnt = static_cast<unsigned*>(scratch(sizeof(unsigned) * NK * G, 8));
-  unsigned* loff = static_cast<unsigned*>(scratch(sizeof(unsigned) * NK * G, 9));
+  unsigned* runs = static_cast<unsigned*>(scratch(sizeof(unsigned) * NK * G, 8));
   unsigned* rowoff = static_cast<unsigned*>(scratch(sizeof(unsigned) * NK * (G + 1), 10));
   unsigned* small = static_cast<unsigned*>(scratch(sizeof(unsigned) * (2 * (NK + 1) + 64), 11));
   unsigned* totals = small;
@@ -672,17 +690,21 @@ void sync_forward_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, 
   double* P0 = pref;
   double* P1 = pref + M1 * NK;
 
-  binned_bucket<<<G, kThreads, bucket_smem(plan), st>>>(D, static_cast<unsigned>(n), P, sortW, sortPhi, cnt, loff);
+  fill_zero<<<static_cast<unsigned>(std::min<size_t>(ceil_div(NK * G, kThreads), 4096)), kThreads, 0, st>>>(runs,
+                                                                                                  NK * G);
+  RGN_HIP(hipGetLastError());
+  stage_mark(st, "fill_zero");
+  binned_bucket<<<G, kThreads, bucket_smem(plan), st>>>(D, static_cast<unsigned>(n), P, sortW, sortPhi, runs);
   RGN_HIP(hipGetLastError());
   stage_mark(st, "binned_bucket");
-  binned_scan_rows<<<plan.NK, kThreads, 0, st>>>(cnt, G, rowoff, totals);
+  binned_scan_rows<<<plan.NK, kThreads, 0, st>>>(runs, G, rowoff, totals);
   RGN_HIP(hipGetLastError());
   stage_mark(st, "binned_scan_rows");
   binned_scan_items<<<1, kThreads, 0, st>>>(totals, plan.NK, istart);
   RGN_HIP(hipGetLastError());
   stage_mark(st, "binned_scan_items");
   binned_subhist<<<static_cast<unsigned>(max_items), kSubThreads, subhist_smem(plan), st>>>(
-    P, G, istart, totals, rowoff, loff, sortW, sortPhi, partial);
+    P, G, istart, totals, rowoff, runs, sortW, sortPhi, partial);
   RGN_HIP(hipGetLastError());
   stage_mark(st, "binned_subhist");
   binned_prefix<<<plan.NK, kPrefThreads, sizeof(double) * 2 * M1, st>>>(plan.M, istart, partial, P0, P1);

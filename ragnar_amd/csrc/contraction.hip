@@ -7,7 +7,8 @@
 // same interpolation coordinate and segment table as the emission kernels
 // (one thread per entry).  The GEMM then runs on the matrix cores:
 //   RGN_PREC_FP32  rocBLAS SGEMM (a plain library GEMM, FP32 throughout);
-//   RGN_PREC_F16X3 / RGN_PREC_F16X1: split-FP16 MFMA kernel (contraction_f16.hip).
+//   RGN_PREC_F16X3 / RGN_PREC_F16X1: split-FP16 MFMA kernel (contraction_f16.hip),
+//   FP32 accumulation with per-block power-of-two scaling.
 #include <rocblas/rocblas.h>
 
 #include <mutex>
@@ -17,10 +18,9 @@
 
 namespace rgn {
 
-void contraction_f16(const SyncPlan&, const float*, int, const float*, size_t, int, float*,
-                     hipStream_t) {
-  throw Error(RGN_ERR_UNSUPPORTED, "split-FP16 contraction is not available in this build");
-}
+// Split-FP16 MFMA form, contraction_f16.hip.
+void contraction_f16(const SyncPlan& plan, const float* d_k, int ng, const float* d_f, size_t nd, int nprod,
+                     float* d_out, hipStream_t st);
 
 namespace {
 

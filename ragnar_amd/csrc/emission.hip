@@ -497,10 +497,13 @@ void launch_combine(const SyncPlan& plan, const double* partials, int grid, floa
   stage_mark(st, "combine_kernel");
 }
 
-// Direct pair-by-pair kernels below ~2^28 evals (the factorised form has a
-// fixed cost of seven launches); the factorised form above.  RAGNAR_SYNC_METHOD
-// (direct | binned) overrides the automatic choice.
-int resolve_method(const SyncPlan& plan, size_t n, int method) {
+// Automatic choice between the direct and the factorised kernels from a cost
+// model measured on MI355X (2^16 .. 2^24 particles, 256 bins):
+//   direct forward  ~ 18 us + n * nbins / 6.2e12 s,
+//   direct adjoint  ~ 20 us + n * nbins / 4.4e12 s,
+//   binned forward  ~ 50 us + n * 15 ps,   binned adjoint ~ 20 us + n * 16 ps.
+// RAGNAR_SYNC_METHOD (direct | binned) overrides the automatic choice.
+int resolve_method(const SyncPlan& plan, size_t n, int method, bool adjoint) {
   // particle indices of the factorised kernels are 32-bit
   const bool binned_fits = plan.binned_ok && n < (size_t{1} << 31);
   if (method == RGN_METHOD_AUTO) {
@@ -508,14 +511,17 @@ int resolve_method(const SyncPlan& plan, size_t n, int method) {
       if (std::string(m) == "direct") return RGN_METHOD_DIRECT;
       if (std::string(m) == "binned" && binned_fits) return RGN_METHOD_BINNED;
     }
-    return binned_fits && static_cast<double>(n) * plan.nbins >= 268435456.0 ? RGN_METHOD_BINNED
-                                                                             : RGN_METHOD_DIRECT;
+    if (!binned_fits) return RGN_METHOD_DIRECT;
+    const double pairs = static_cast<double>(n) * plan.nbins;
+    const double t_direct = adjoint ? 20e-6 + pairs / 4.4e12 : 18e-6 + pairs / 6.2e12;
+    const double t_binned = adjoint ? 20e-6 + n * 16e-12 : 50e-6 + n * 15e-12;
+    return t_binned < t_direct ? RGN_METHOD_BINNED : RGN_METHOD_DIRECT;
   }
   RGN_REQUIRE(method == RGN_METHOD_DIRECT || method == RGN_METHOD_BINNED, RGN_ERR_INVALID,
               "unknown method");
   RGN_REQUIRE(method != RGN_METHOD_BINNED || binned_fits, RGN_ERR_UNSUPPORTED,
-              "the binned method needs fewer than 2^31 particles, at most 4096 distinct bin "
-              "offsets and a bin range of at most 4096 table units");
+              "the binned method needs fewer than 2^31 particles, at most 1024 distinct bin "
+              "offsets and a bin range of at most 3072 table units");
   return method;
 }
 
@@ -565,7 +571,7 @@ int rgn_sync_spectrum_method(const rgn_sync_plan* plan, const float* const d_u[3
     RGN_REQUIRE(plan != nullptr, RGN_ERR_INVALID, "null plan");
     RGN_REQUIRE(d_out != nullptr, RGN_ERR_INVALID, "null output");
     const ParticlesDev D = make_particles_checked(d_u, d_e, d_b, d_w, n);
-    if (resolve_method(plan->p, n, method) == RGN_METHOD_BINNED) {
+    if (resolve_method(plan->p, n, method, false) == RGN_METHOD_BINNED) {
       sync_forward_binned(plan->p, D, n, d_out, as_stream(stream));
     } else {
       // 16 blocks per CU measured fastest on MI355X (2^24 x 256: 0.68 ms vs 0.74 at 4)
@@ -597,7 +603,7 @@ int rgn_sync_adjoint_method(const rgn_sync_plan* plan, const float* const d_u[3]
     RGN_REQUIRE(plan != nullptr, RGN_ERR_INVALID, "null plan");
     RGN_REQUIRE(d_gbar != nullptr, RGN_ERR_INVALID, "null upstream gradient");
     const ParticlesDev D = make_particles_checked(d_u, d_e, d_b, d_w, n);
-    if (resolve_method(plan->p, n, method) == RGN_METHOD_BINNED) {
+    if (resolve_method(plan->p, n, method, true) == RGN_METHOD_BINNED) {
       sync_adjoint_binned(plan->p, D, n, d_gbar, d_gw, d_gu, d_gb, d_ge, as_stream(stream));
       return;
     }

@@ -86,7 +86,7 @@ void sync_forward_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, 
 void sync_adjoint_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, const float* gbar,
                          float* gw, float* const gu[3], float* const gb[3], float* const ge[3],
                          hipStream_t st);
-int resolve_method(const SyncPlan& plan, size_t n, int method);
+int resolve_method(const SyncPlan& plan, size_t n, int method, bool adjoint);
 
 PlanDev device_view(const SyncPlan& plan);
 size_t forward_smem(const SyncPlan& plan);

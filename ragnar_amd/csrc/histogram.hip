@@ -124,7 +124,7 @@ __global__ void __launch_bounds__(kThreads) hist_kernel(const float* __restrict_
                                                         const double* __restrict__ g_edges, int nb,
                                                         float emin, float emax, float scale,
                                                         int log_bins, int fourvel, int vec,
-                                                        unsigned long long* __restrict__ counts,
+                                                        unsigned* __restrict__ cpart,
                                                         double* __restrict__ partials) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int wave = threadIdx.x >> 6;
@@ -177,26 +177,42 @@ __global__ void __launch_bounds__(kThreads) hist_kernel(const float* __restrict_
   for (int i = threadIdx.x; i < nb; i += kThreads) {
     const unsigned c = lcnt[i] + lcnt[nb + i] + lcnt[2 * nb + i] + lcnt[3 * nb + i];
     const double s = ((lsum[i] + lsum[nb + i]) + lsum[2 * nb + i]) + lsum[3 * nb + i];
-    if (counts && c) atomicAdd(counts + i, static_cast<unsigned long long>(c));
+    cpart[static_cast<size_t>(i) * gridDim.x + blockIdx.x] = c;
     partials[static_cast<size_t>(i) * gridDim.x + blockIdx.x] = s;
   }
 }
 
+// Fixed-order sums of the per-block partials: FP64 sums and (when counts is
+// not null) uint64 counts.  No global atomics.
 __global__ void __launch_bounds__(kThreads) hist_combine_kernel(const double* __restrict__ partials,
+                                                                const unsigned* __restrict__ cpart,
                                                                 int nblocks, int nb,
-                                                                float* __restrict__ out) {
+                                                                float* __restrict__ out,
+                                                                unsigned long long* __restrict__ counts) {
   const int i = blockIdx.x;
   if (i >= nb) return;
   double s = 0.0;
-  for (int b = threadIdx.x; b < nblocks; b += kThreads) s += partials[static_cast<size_t>(i) * nblocks + b];
+  unsigned long long c = 0;
+  for (int b = threadIdx.x; b < nblocks; b += kThreads) {
+    s += partials[static_cast<size_t>(i) * nblocks + b];
+    c += cpart[static_cast<size_t>(i) * nblocks + b];
+  }
   __shared__ double red[kThreads];
+  __shared__ unsigned long long cred[kThreads];
   red[threadIdx.x] = s;
+  cred[threadIdx.x] = c;
   __syncthreads();
   for (int w = kThreads / 2; w > 0; w >>= 1) {
-    if (static_cast<int>(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
+    if (static_cast<int>(threadIdx.x) < w) {
+      red[threadIdx.x] += red[threadIdx.x + w];
+      cred[threadIdx.x] += cred[threadIdx.x + w];
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[i] = static_cast<float>(red[0]);
+  if (threadIdx.x == 0) {
+    out[i] = static_cast<float>(red[0]);
+    if (counts) counts[i] = cred[0];
+  }
 }
 
 }  // namespace
@@ -262,16 +278,19 @@ int rgn_energy_hist(const rgn_hist_plan* plan, const float* const d_u[3], size_t
     const size_t per_thread = vec ? 8 : 1;
     const size_t tiles = ceil_div(std::max<size_t>(n, 1), kThreads * per_thread);
     const int grid = static_cast<int>(std::min<size_t>(tiles, static_cast<size_t>(num_cus()) * 8));
-    double* partials = static_cast<double*>(scratch(sizeof(double) * nb * static_cast<size_t>(grid), 2));
-    if (d_counts) RGN_HIP(hipMemsetAsync(d_counts, 0, sizeof(uint64_t) * nb, st));
+    const size_t np = static_cast<size_t>(nb) * grid;
+    double* partials = static_cast<double*>(scratch((sizeof(double) + sizeof(unsigned)) * np, 2));
+    unsigned* cpart = reinterpret_cast<unsigned*>(partials + np);
     const size_t smem = 4 * nb * (sizeof(double) + sizeof(unsigned)) + (nb + 1) * sizeof(double);
     hist_kernel<<<grid, kThreads, smem, st>>>(
       d_u[0], d_u[1], d_u[2], n, plan->d_edges.as<double>(), nb, plan->emin, plan->emax,
-      plan->approx_scale, plan->log_bins, fourvel ? 1 : 0, vec,
-      reinterpret_cast<unsigned long long*>(d_counts), partials);
+      plan->approx_scale, plan->log_bins, fourvel ? 1 : 0, vec, cpart, partials);
     RGN_HIP(hipGetLastError());
-    hist_combine_kernel<<<nb, kThreads, 0, st>>>(partials, grid, nb, d_hist);
+    stage_mark(st, "hist_kernel");
+    hist_combine_kernel<<<nb, kThreads, 0, st>>>(partials, cpart, grid, nb, d_hist,
+                                                 reinterpret_cast<unsigned long long*>(d_counts));
     RGN_HIP(hipGetLastError());
+    stage_mark(st, "hist_combine_kernel");
   });
 }
 

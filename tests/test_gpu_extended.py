@@ -214,3 +214,34 @@ def test_adjoint_methods_agree_large(orc, table):
         assert np.percentile(r, 99.9) < 1e-6
         assert (r > 1e-5).mean() < 1e-4
         assert r.max() < 1e-2
+
+
+@pytest.mark.parametrize("precision,tol", [(1, 2e-5), (2, 3e-3)])
+def test_contraction_split_fp16_rows(orc, table, precision, tol):
+    """Split-FP16 matrix-core contraction (F16X3 = 1, F16X1 = 2) against the
+    oracle's FP64 FromDist of each row, on sizes that are not multiples of the
+    128 x 128 x 64 tiles.  Bins above 1e-6 of a row's peak must be within tol;
+    the error on fainter bins is printed (a g block straddling a steep cutoff
+    keeps 22 bits only within 2^18 of its largest value)."""
+    nd, ng, nb = 300, 500, 200
+    f, _, _ = workloads.distributions(nd=nd, ng=ng, seed=7)
+    g = orc.logspace(1, 1e6, ng)
+    bins = orc.logspace(1e-2, 1e7, nb)
+    plan = T.SyncPlan(table, bins)
+    S = T.sync_fromdist_batched(plan, dev(g), dev(f), True, precision).cpu().numpy().astype(np.float64)
+    S32 = T.sync_fromdist_batched(plan, dev(g), dev(f), True, 0).cpu().numpy().astype(np.float64)
+    worst = {1e-6: 0.0, 1e-9: 0.0, 1e-12: 0.0}
+    worst32 = 0.0
+    for d in range(0, nd, 7):
+        _, f64 = orc.sync_fromdist(g, f[d], True, bins, mode=orc.FP64, want_fp64=True)
+        peak = f64.max()
+        for thr in worst:
+            live = f64 > thr * peak
+            worst[thr] = max(worst[thr], float(np.max(np.abs(S[d][live] / f64[live] - 1))))
+        live = f64 > 1e-6 * peak
+        worst32 = max(worst32, float(np.max(np.abs(S32[d][live] / f64[live] - 1))))
+    print("split-fp16 precision", precision, "max rel err vs FP64 by threshold:", worst,
+          "fp32 path:", worst32)
+    assert worst[1e-6] <= tol
+    # the last partial row tile and column tile are written, nothing beyond
+    assert np.all(np.isfinite(S)) and np.any(S[-1] > 0) and np.any(S[:, -1] > 0)
