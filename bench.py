@@ -193,6 +193,39 @@ def read_traffic(kernel):
     return None
 
 
+def bench_fit(T, workloads, orc, torch, table, bins):
+    """2^26 particles (same generator as the headline step), weights
+    W = |u|^(p + 2.5) exp(-|u| / c) at p = -2.2, c = 3e3, loss
+    sum_e (log S_e - log S_e^target)^2 over the 256 bins: one step is the
+    forward, the loss gradient dL/dS and the adjoint (dL/dw, dL/du, dL/dB),
+    each with the library's default method."""
+    n = 1 << 26
+    nb = len(bins)
+    u_h, b_h = workloads.particles(n, seed=321)
+    w_h = workloads.fit_weights(u_h)
+    U = torch.from_numpy(u_h).cuda()
+    B = torch.from_numpy(b_h).cuda()
+    w = torch.from_numpy(w_h).cuda()
+    del u_h, b_h, w_h
+    plan = T.SyncPlan(table, bins)
+    target = (T.sync_spectrum(plan, U, B, w) * 1.05).clamp_min(1e-30).log()
+
+    def fwd():
+        return T.sync_spectrum(plan, U, B, w)
+
+    def step():
+        S = fwd()
+        r = S.clamp_min(1e-30).log() - target
+        gbar = torch.where(S > 0, 2.0 * r / S.clamp_min(1e-30), torch.zeros_like(S))
+        T.sync_adjoint(plan, U, B, w, None, gbar, need=("w", "u", "b"))
+
+    ms_f = event_time(fwd, 20, 3, torch)
+    ms_fa = event_time(step, 20, 3, torch)
+    evals = n * nb
+    return {"particles": n, "ms_fwd": ms_f, "ms_fwd_adj": ms_fa, "ratio": ms_fa / ms_f,
+            "fwd_adj_evals_per_s": evals / (ms_fa * 1e-3)}
+
+
 def bench_cells(T, workloads, orc, torch, table):
     """4096 cells, lognormal counts (median 4000, sigma 1): one persistent launch
     in a hipGraph against 4096 calls of the single-population form."""
@@ -426,6 +459,11 @@ def main():
                   for m in ("direct", "binned")}
         extras["fwd_adj"] = {"ms": ms_fa, "evals_per_s": evals / (ms_fa * 1e-3),
                              "ratio_to_fwd": ms_fa / ms, "adjoint_ms_by_method": adj_ms}
+        # differentiable fit at 2^26 particles (fwd + adjoint of the reweighting loss)
+        try:
+            extras["fit_2p26"] = bench_fit(T, workloads, orc, torch, table, bins)
+        except Exception as exc:  # reported, the headline line still prints
+            extras["fit_2p26"] = {"error": repr(exc)}
         # energy histogram (HBM bound: 12 B per particle)
         hbins = orc.logspace(1e-2, 1e3, 200)
         hplan = T.HistPlan(hbins, True)

@@ -13,6 +13,17 @@ from pathlib import Path
 
 _HERE = Path(__file__).resolve().parent
 
+# One HIP runtime per process.  PyTorch-ROCm ships its own libamdhip64 and its
+# libraries ask for it by the unversioned name, which does not match the
+# versioned name this package links; so if this package loaded the system
+# runtime first, importing torch afterwards would load a second runtime and
+# the device would be unusable from ours.  Loading torch first makes both use
+# torch's runtime.
+try:
+    import torch  # noqa: F401
+except ImportError:  # pragma: no cover - torch is optional for the ragnar module
+    pass
+
 try:
     from . import ragnar  # noqa: F401  (pybind11 module, links libragnar_hip.so)
 except ImportError as exc:  # pragma: no cover - exercised only when unbuilt

@@ -118,6 +118,7 @@ constexpr int kPer = 16;                   // particles per thread in the bucket
 constexpr int kChunk = kThreads * kPer;    // particles per bucket block
 constexpr unsigned kItem = 16384;          // sources per sub-histogram work item
 constexpr int kSubThreads = 512;           // threads of a sub-histogram work item
+constexpr int kBatch = 4;                  // sources per lane with loads in flight together
 constexpr int kWin = 2048;                 // runs per LDS window of a work item
 constexpr int kPrefThreads = 1024;         // 4 item groups x 256 sub-cells per prefix block
 
@@ -261,30 +262,60 @@ __global__ void __launch_bounds__(kThreads) binned_bucket(ParticlesDev D, unsign
   }
 }
 
-// Run lengths of one bucket's row of the run table.
-struct RunLengths {
-  const unsigned* r;
-  __device__ unsigned operator[](int i) const { return run_len(r[i]); }
-};
-
-// Pass 2a: per bucket k, rowoff[k * (G + 1) + b] = sum of the run lengths of
-// blocks b' < b; rowoff[k * (G + 1) + G] = totals[k].
+// Pass 2a: per bucket k, its non-empty runs in block order, compacted:
+//   cvoff[k * (G + 1) + i]: virtual offset of run i within the bucket
+//                           (cvoff[k * (G + 1) + nrun[k]] = totals[k]);
+//   cidx[k * G + i]:        physical index of the run's first source.
+// One exclusive scan over blocks of (run length, non-empty flag) packed in 64 bits.
 __global__ void __launch_bounds__(kThreads) binned_scan_rows(const unsigned* __restrict__ runs, int G,
-                                                             unsigned* __restrict__ rowoff,
+                                                             unsigned* __restrict__ cvoff,
+                                                             unsigned* __restrict__ cidx,
+                                                             unsigned* __restrict__ nrun,
                                                              unsigned* __restrict__ totals) {
-  __shared__ unsigned part[kThreads];
+  __shared__ unsigned long long part[kThreads];
   const int k = blockIdx.x;
-  unsigned* row = rowoff + static_cast<size_t>(k) * (G + 1);
-  const unsigned total =
-    block_scan_exclusive<unsigned>(RunLengths{runs + static_cast<size_t>(k) * G}, row, G, part);
-  if (threadIdx.x == 0) {
-    row[G] = total;
+  const unsigned* row = runs + static_cast<size_t>(k) * G;
+  unsigned* vo = cvoff + static_cast<size_t>(k) * (G + 1);
+  unsigned* ci = cidx + static_cast<size_t>(k) * G;
+  const int per = (G + kThreads - 1) / kThreads;
+  const int i0 = min(G, static_cast<int>(threadIdx.x) * per), i1 = min(G, i0 + per);
+  unsigned long long acc = 0;
+  for (int i = i0; i < i1; ++i) {
+    const unsigned len = run_len(row[i]);
+    acc += len + (len ? (1ull << 32) : 0ull);
+  }
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  for (int d = 1; d < kThreads; d <<= 1) {
+    const unsigned long long v = static_cast<int>(threadIdx.x) >= d ? part[threadIdx.x - d] : 0ull;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  acc = threadIdx.x ? part[threadIdx.x - 1] : 0ull;
+  for (int i = i0; i < i1; ++i) {
+    const unsigned r = row[i];
+    const unsigned len = run_len(r);
+    if (len) {
+      const unsigned c = static_cast<unsigned>(acc >> 32);
+      vo[c] = static_cast<unsigned>(acc);
+      ci[c] = static_cast<unsigned>(i) * kChunk + run_start(r);
+      acc += len + (1ull << 32);
+    }
+  }
+  if (threadIdx.x == kThreads - 1) {
+    const unsigned nr = static_cast<unsigned>(part[kThreads - 1] >> 32);
+    const unsigned total = static_cast<unsigned>(part[kThreads - 1]);
+    vo[nr] = total;
+    nrun[k] = nr;
     totals[k] = total;
   }
 }
 
 // Pass 2b: work items per bucket (one block): istart[k] = first item of bucket
-// k, istart[NK] = number of items.
+// k, istart[NK] = number of items.  A separate launch: folding it into the
+// last block of pass 2a needs an agent-scope release per block, which writes
+// back the XCD's L2 and measured slower.
 __global__ void __launch_bounds__(kThreads) binned_scan_items(const unsigned* __restrict__ totals, int NK,
                                                               unsigned* __restrict__ istart) {
   __shared__ unsigned items[kBinnedMaxNK];
@@ -301,8 +332,9 @@ __global__ void __launch_bounds__(kThreads) binned_scan_items(const unsigned* __
 // coalesced, four loads in flight per lane.
 __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G, const unsigned* __restrict__ istart,
                                                               const unsigned* __restrict__ totals,
-                                                              const unsigned* __restrict__ rowoff,
-                                                              const unsigned* __restrict__ runs,
+                                                              const unsigned* __restrict__ cvoff,
+                                                              const unsigned* __restrict__ cidx,
+                                                              const unsigned* __restrict__ nrun,
                                                               const float* __restrict__ sortW,
                                                               const float* __restrict__ sortPhi,
                                                               double* __restrict__ partial) {
@@ -316,7 +348,6 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
   unsigned* wlo = wro + kWin + 1;                              // kWin
   unsigned short* jlut = reinterpret_cast<unsigned short*>(wlo + kWin);
   __shared__ unsigned s_cnt[3];
-  __shared__ unsigned s_part[kSubThreads];
   const unsigned it = blockIdx.x;
   if (it >= istart[P.NK]) return;  // uniform per block
   if (threadIdx.x < 3) s_cnt[threadIdx.x] = 0u;
@@ -338,13 +369,15 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
   const int k = static_cast<int>(s_cnt[0]) - 1;
   const unsigned v0 = (it - istart[k]) * kItem;
   const unsigned v1 = min(totals[k], v0 + kItem);
-  const unsigned* row = rowoff + static_cast<size_t>(k) * (G + 1);
-  const unsigned* lrow = runs + static_cast<size_t>(k) * G;
-  // b_lo = #{b < G : row[b] <= v0} - 1, b_hi = #{b < G : row[b] <= v1 - 1} - 1
+  const unsigned* vo = cvoff + static_cast<size_t>(k) * (G + 1);
+  const unsigned* ci = cidx + static_cast<size_t>(k) * G;
+  const int nr = static_cast<int>(nrun[k]);
+  // first and last run of the item: #{i < nr : vo[i] <= v} - 1 (vo strictly
+  // increases: every listed run is non-empty)
   {
     unsigned c0 = 0, c1 = 0;
-    for (int q = threadIdx.x; q < G; q += kSubThreads) {
-      const unsigned r = row[q];
+    for (int q = threadIdx.x; q < nr; q += kSubThreads) {
+      const unsigned r = vo[q];
       c0 += r <= v0;
       c1 += r <= v1 - 1;
     }
@@ -352,46 +385,21 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
     if (c1) atomicAdd(&s_cnt[2], c1);
   }
   __syncthreads();
-  const int blo = static_cast<int>(s_cnt[1]) - 1, bhi = static_cast<int>(s_cnt[2]) - 1;
+  const int rlo = static_cast<int>(s_cnt[1]) - 1, rhi = static_cast<int>(s_cnt[2]) - 1;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int ws = blo; ws <= bhi; ws += kWin) {
-    const int nw = min(bhi - ws + 1, kWin);
-    // the window's runs clipped to [v0, v1), empty ones dropped:
-    // run c covers virtual [crs[c], crs[c + 1]) from physical index cidx[c]
-    constexpr int kPerT = kWin / kSubThreads;
-    unsigned fa[kPerT], fidx[kPerT];
-    unsigned nfl = 0;
-#pragma unroll
-    for (int q = 0; q < kPerT; ++q) {
-      const int i = threadIdx.x * kPerT + q;
-      fa[q] = 0u;
-      fidx[q] = ~0u;
-      if (i < nw) {
-        const unsigned r0 = row[ws + i];
-        const unsigned ra = max(r0, v0), rz = min(row[ws + i + 1], v1);
-        if (ra < rz) {
-          fa[q] = ra;
-          fidx[q] = static_cast<unsigned>(ws + i) * kChunk + run_start(lrow[ws + i]) + (ra - r0);
-          ++nfl;
-        }
-      }
+  for (int ws = rlo; ws <= rhi; ws += kWin) {
+    const int nc = min(rhi - ws + 1, kWin);
+    // the window's runs clipped to [v0, v1): run c covers virtual
+    // [crs[c], crs[c + 1]) from physical index cix[c]
+    unsigned* crs = wro;   // nc + 1 entries
+    unsigned* cix = wlo;   // nc entries
+    for (int c = threadIdx.x; c < nc; c += kSubThreads) {
+      const unsigned r0 = vo[ws + c];
+      const unsigned ra = max(r0, v0);
+      crs[c] = ra;
+      cix[c] = ci[ws + c] + (ra - r0);
     }
-    wro[threadIdx.x] = nfl;
-    __syncthreads();
-    const int nc = static_cast<int>(block_scan_exclusive<unsigned, kSubThreads>(wro, wro, kSubThreads, s_part));
-    unsigned c = wro[threadIdx.x];
-    __syncthreads();  // wro is rewritten below
-    unsigned* crs = wro;  // nc + 1 entries
-    unsigned* cidx = wlo;  // nc entries
-#pragma unroll
-    for (int q = 0; q < kPerT; ++q) {
-      if (fidx[q] != ~0u) {
-        crs[c] = fa[q];
-        cidx[c] = fidx[q];
-        ++c;
-      }
-    }
-    if (threadIdx.x == 0) crs[nc] = min(row[ws + nw], v1);
+    if (threadIdx.x == 0) crs[nc] = min(vo[ws + nc], v1);
     __syncthreads();
     if (nc > 0) {
       // wave w walks positions [pa, pb) of the window, lane-interleaved
@@ -407,15 +415,15 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
         }
         cur = lo;
       }
-      for (unsigned v00 = pa; v00 < pb; v00 += 256) {
-        unsigned idx[4];
-        bool ok[4];
+      // kBatch positions per lane at a time: resolve them through the run
+      // cursor, issue every load, then do the sub-cell lookups and atomics
+      for (unsigned v00 = pa; v00 < pb; v00 += 64 * kBatch) {
+        unsigned idx[kBatch];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < kBatch; ++q) {
           const unsigned v = v00 + q * 64 + lane;
-          ok[q] = v < pb;
-          idx[q] = 0u;
-          if (ok[q]) {
+          idx[q] = ~0u;
+          if (v < pb) {
             if (crs[cur + 1] <= v) {
               ++cur;
               if (crs[cur + 1] <= v) {
@@ -427,18 +435,18 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
                 cur = lo;
               }
             }
-            idx[q] = cidx[cur] + (v - crs[cur]);
+            idx[q] = cix[cur] + (v - crs[cur]);
           }
         }
-        float w[4], ph[4];
+        float w[kBatch], ph[kBatch];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          w[q] = ok[q] ? sortW[idx[q]] : 0.0f;
-          ph[q] = ok[q] ? sortPhi[idx[q]] : 0.0f;
+        for (int q = 0; q < kBatch; ++q) {
+          w[q] = idx[q] != ~0u ? sortW[idx[q]] : 0.0f;
+          ph[q] = idx[q] != ~0u ? sortPhi[idx[q]] : 0.0f;
         }
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          if (ok[q]) {
+        for (int q = 0; q < kBatch; ++q) {
+          if (idx[q] != ~0u) {
             const double wd = w[q], phd = ph[q];
             const int j = subcell(alpha, jlut, P.M, P.JQ, phd);
             atomicAdd(m0 + j, wd);
@@ -626,7 +634,7 @@ __global__ void __launch_bounds__(kThreads) binned_adj_gather(ParticlesDev D, si
     float W;
     double B;
     ParticleGrads g{};
-    const OmegaTerms o = omega_terms(cur, P.em);
+    const OmegaTerms o = omega_terms_approx(cur, P.em);
     if (particle_wb(o, cur, P, W, B)) {
       const double fl = floor(B);
       const double phi = B - fl;
@@ -681,30 +689,32 @@ void sync_forward_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, 
   float* sortW = static_cast<float*>(scratch(sizeof(float) * G * size_t{kChunk}, 6));
   float* sortPhi = static_cast<float*>(scratch(sizeof(float) * G * size_t{kChunk}, 7));
   unsigned* runs = static_cast<unsigned*>(scratch(sizeof(unsigned) * NK * G, 8));
-  unsigned* rowoff = static_cast<unsigned*>(scratch(sizeof(unsigned) * NK * (G + 1), 10));
-  unsigned* small = static_cast<unsigned*>(scratch(sizeof(unsigned) * (2 * (NK + 1) + 64), 11));
+  unsigned* cvoff = static_cast<unsigned*>(scratch(sizeof(unsigned) * NK * (G + 1), 10));
+  unsigned* cidx = static_cast<unsigned*>(scratch(sizeof(unsigned) * NK * G, 9));
+  unsigned* small = static_cast<unsigned*>(scratch(sizeof(unsigned) * (3 * (NK + 1) + 64), 11));
   unsigned* totals = small;
   unsigned* istart = small + (NK + 1);
+  unsigned* nrun = small + 2 * (NK + 1);
   double* partial = static_cast<double*>(scratch(sizeof(double) * 2 * M1 * max_items, 12));
   double* pref = static_cast<double*>(scratch(sizeof(double) * 2 * M1 * NK, 13));
   double* P0 = pref;
   double* P1 = pref + M1 * NK;
 
-  fill_zero<<<static_cast<unsigned>(std::min<size_t>(ceil_div(NK * G, kThreads), 4096)), kThreads, 0, st>>>(runs,
-                                                                                                  NK * G);
+  fill_zero<<<static_cast<unsigned>(std::min<size_t>(ceil_div(NK * G, kThreads), 4096)), kThreads, 0, st>>>(
+    runs, NK * G);
   RGN_HIP(hipGetLastError());
   stage_mark(st, "fill_zero");
   binned_bucket<<<G, kThreads, bucket_smem(plan), st>>>(D, static_cast<unsigned>(n), P, sortW, sortPhi, runs);
   RGN_HIP(hipGetLastError());
   stage_mark(st, "binned_bucket");
-  binned_scan_rows<<<plan.NK, kThreads, 0, st>>>(runs, G, rowoff, totals);
+  binned_scan_rows<<<plan.NK, kThreads, 0, st>>>(runs, G, cvoff, cidx, nrun, totals);
   RGN_HIP(hipGetLastError());
   stage_mark(st, "binned_scan_rows");
   binned_scan_items<<<1, kThreads, 0, st>>>(totals, plan.NK, istart);
   RGN_HIP(hipGetLastError());
   stage_mark(st, "binned_scan_items");
   binned_subhist<<<static_cast<unsigned>(max_items), kSubThreads, subhist_smem(plan), st>>>(
-    P, G, istart, totals, rowoff, runs, sortW, sortPhi, partial);
+    P, G, istart, totals, cvoff, cidx, nrun, sortW, sortPhi, partial);
   RGN_HIP(hipGetLastError());
   stage_mark(st, "binned_subhist");
   binned_prefix<<<plan.NK, kPrefThreads, sizeof(double) * 2 * M1, st>>>(plan.M, istart, partial, P0, P1);

@@ -498,10 +498,11 @@ void launch_combine(const SyncPlan& plan, const double* partials, int grid, floa
 }
 
 // Automatic choice between the direct and the factorised kernels from a cost
-// model measured on MI355X (2^16 .. 2^24 particles, 256 bins):
-//   direct forward  ~ 18 us + n * nbins / 6.2e12 s,
-//   direct adjoint  ~ 20 us + n * nbins / 4.4e12 s,
-//   binned forward  ~ 50 us + n * 15 ps,   binned adjoint ~ 20 us + n * 16 ps.
+// model measured on MI355X (2^16 .. 2^24 particles, 32 .. 512 bins):
+//   direct forward  ~ 18 us + n * c(NB), c = 22, 29, 35, 41, 55, 70, 82, 95 ps
+//                     for NB = 1 .. 8 bins per lane (64 NB bins per wave),
+//   direct adjoint  ~ 25 us + n * nbins / 4.6e12 s,
+//   binned forward  ~ 60 us + n * 13.5 ps,   binned adjoint ~ 28 us + n * 14 ps.
 // RAGNAR_SYNC_METHOD (direct | binned) overrides the automatic choice.
 int resolve_method(const SyncPlan& plan, size_t n, int method, bool adjoint) {
   // particle indices of the factorised kernels are 32-bit
@@ -512,9 +513,11 @@ int resolve_method(const SyncPlan& plan, size_t n, int method, bool adjoint) {
       if (std::string(m) == "binned" && binned_fits) return RGN_METHOD_BINNED;
     }
     if (!binned_fits) return RGN_METHOD_DIRECT;
-    const double pairs = static_cast<double>(n) * plan.nbins;
-    const double t_direct = adjoint ? 20e-6 + pairs / 4.4e12 : 18e-6 + pairs / 6.2e12;
-    const double t_binned = adjoint ? 20e-6 + n * 16e-12 : 50e-6 + n * 15e-12;
+    static const double c_nb[9] = {0, 22e-12, 29e-12, 35e-12, 41e-12, 55e-12, 70e-12, 82e-12, 95e-12};
+    const double nd = static_cast<double>(n);
+    const double t_direct = adjoint ? 25e-6 + nd * plan.nbins / 4.6e12
+                                    : 18e-6 + nd * c_nb[std::min(std::max(plan.NB, 1), 8)];
+    const double t_binned = adjoint ? 28e-6 + nd * 14e-12 : 60e-6 + nd * 13.5e-12;
     return t_binned < t_direct ? RGN_METHOD_BINNED : RGN_METHOD_DIRECT;
   }
   RGN_REQUIRE(method == RGN_METHOD_DIRECT || method == RGN_METHOD_BINNED, RGN_ERR_INVALID,

@@ -141,6 +141,52 @@ __device__ __forceinline__ OmegaTerms omega_terms(const RawParticle& r, const Em
   return o;
 }
 
+// 1 / x to about one ulp: hardware estimate and two Newton steps.
+__device__ __forceinline__ double rcp_nr(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  y = fma(y, fma(-x, y, 1.0), y);
+  return fma(y, fma(-x, y, 1.0), y);
+}
+
+// sqrt(x) to about one ulp for finite x >= 0 (hardware reciprocal square root
+// estimate and two Goldschmidt steps); x = 0 gives 0, negative x gives NaN.
+__device__ __forceinline__ double sqrt_nr(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = 0.5 * y;
+  double r = fma(-g, h, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  r = fma(-g, h, 0.5);
+  g = fma(g, r, g);
+  return x > 0.0 ? g : (x == 0.0 ? 0.0 : __builtin_nan(""));
+}
+
+// The OmegaSync terms to about one ulp per operation, for the adjoint's
+// gradients (which need no correct rounding): reciprocal products instead of
+// divisions, refined square roots.  A particle's e_peak may differ from the
+// reference's in the last float bit; the forward kernels use omega_terms.
+__device__ __forceinline__ OmegaTerms omega_terms_approx(const RawParticle& r, const EmitConst& C) {
+  OmegaTerms o;
+  const float ux = r.ux, uy = r.uy, uz = r.uz, ex = r.ex, ey = r.ey, ez = r.ez;
+  const float bx = r.bx, by = r.by, bz = r.bz;
+  const double gamma = sqrt_nr(1.0 + ux * ux + uy * uy + uz * uz);
+  const double ig = rcp_nr(gamma);
+  o.gamma = gamma;
+  o.ig = ig;
+  o.btx = ux * ig;
+  o.bty = uy * ig;
+  o.btz = uz * ig;
+  o.bde = o.btx * ex + o.bty * ey + o.btz * ez;
+  o.vx = ex + (o.bty * bz - o.btz * by);
+  o.vy = ey + (o.btz * bx - o.btx * bz);
+  o.vz = ez + (o.btx * by - o.bty * bx);
+  const double v_sqr = o.vx * o.vx + o.vy * o.vy + o.vz * o.vz;
+  o.rootQ = sqrt_nr(v_sqr - o.bde * o.bde);
+  o.chiR = static_cast<float>(o.rootQ * C.rB0);
+  o.e_peak = static_cast<float>(C.e_at * gamma * gamma * o.chiR * C.rDg);
+  return o;
+}
+
 // log2 of a positive finite float, accurate to ~1e-15 absolute:
 // x = m 2^E with m in [sqrt(1/2), sqrt(2)), log2 m = (2 / ln 2) atanh(z),
 // z = (m - 1) / (m + 1), |z| <= 0.1716, odd series through z^15 (truncation
@@ -234,12 +280,6 @@ struct ParticleGrads {
   double gw, gu[3], gb[3], ge[3];
 };
 
-// 1 / x to about one ulp: hardware estimate and two Newton steps.
-__device__ __forceinline__ double rcp_nr(double x) {
-  double y = __builtin_amdgcn_rcp(x);
-  y = fma(y, fma(-x, y, 1.0), y);
-  return fma(y, fma(-x, y, 1.0), y);
-}
 
 __device__ __forceinline__ ParticleGrads emission_grads(const OmegaTerms& o, const RawParticle& r,
                                                         double A, double Cl, const EmitConst& C) {

@@ -123,23 +123,28 @@ __global__ void __launch_bounds__(kThreads) hist_kernel(const float* __restrict_
                                                         const float* __restrict__ uz, size_t n,
                                                         const double* __restrict__ g_edges, int nb,
                                                         float emin, float emax, float scale,
-                                                        int log_bins, int fourvel, int vec,
+                                                        int log_bins, int fourvel, int vec, int qper,
                                                         unsigned* __restrict__ cpart,
                                                         double* __restrict__ partials) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int wave = threadIdx.x >> 6;
-  double* lsum = reinterpret_cast<double*>(smem);                                 // [4][nb]
-  double* ledges = lsum + 4 * nb;                                                 // [nb + 1]
-  unsigned* lcnt = reinterpret_cast<unsigned*>(ledges + nb + 1);                  // [4][nb]
-  for (int i = threadIdx.x; i < 4 * nb; i += kThreads) {
+  // 4 * qper private sub-histograms per block: one per wave and lane group of
+  // 64 / qper lanes (fewer same-address atomics where energies pile up in a
+  // few bins); pitch nb + 1 staggers the copies over the LDS banks
+  const int copies = 4 * qper, pitch = nb + 1;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double* lsum = reinterpret_cast<double*>(smem);                                 // [copies][pitch]
+  double* ledges = lsum + copies * pitch;                                         // [nb + 1]
+  unsigned* lcnt = reinterpret_cast<unsigned*>(ledges + nb + 1);                  // [copies][pitch]
+  for (int i = threadIdx.x; i < copies * pitch; i += kThreads) {
     lsum[i] = 0.0;
     lcnt[i] = 0u;
   }
   for (int i = threadIdx.x; i <= nb; i += kThreads) ledges[i] = g_edges[i];
   __syncthreads();
   const HistParams H{ledges, nb, emin, emax, scale, log_bins, fourvel};
-  double* my_sum = lsum + wave * nb;
-  unsigned* my_cnt = lcnt + wave * nb;
+  const int copy = wave * qper + lane / (64 / qper);
+  double* my_sum = lsum + copy * pitch;
+  unsigned* my_cnt = lcnt + copy * pitch;
   const size_t stride = static_cast<size_t>(gridDim.x) * kThreads;
   const size_t tid = blockIdx.x * static_cast<size_t>(kThreads) + threadIdx.x;
   size_t scalar_from = 0;
@@ -175,8 +180,12 @@ __global__ void __launch_bounds__(kThreads) hist_kernel(const float* __restrict_
   }
   __syncthreads();
   for (int i = threadIdx.x; i < nb; i += kThreads) {
-    const unsigned c = lcnt[i] + lcnt[nb + i] + lcnt[2 * nb + i] + lcnt[3 * nb + i];
-    const double s = ((lsum[i] + lsum[nb + i]) + lsum[2 * nb + i]) + lsum[3 * nb + i];
+    unsigned c = 0;
+    double s = 0.0;
+    for (int q = 0; q < copies; ++q) {  // fixed order
+      c += lcnt[q * pitch + i];
+      s += lsum[q * pitch + i];
+    }
     cpart[static_cast<size_t>(i) * gridDim.x + blockIdx.x] = c;
     partials[static_cast<size_t>(i) * gridDim.x + blockIdx.x] = s;
   }
@@ -226,8 +235,10 @@ int rgn_hist_plan_create(const float* h_bins, int nb, int log_bins, rgn_hist_pla
   return guarded([&] {
     RGN_REQUIRE(out != nullptr && h_bins != nullptr, RGN_ERR_INVALID, "null argument");
     RGN_REQUIRE(nb >= 1, RGN_ERR_INVALID, "no energy bins");
-    RGN_REQUIRE(static_cast<size_t>(nb) * 4 * (sizeof(double) + sizeof(unsigned)) <= 150 * 1024,
-                RGN_ERR_UNSUPPORTED, "too many energy bins for the LDS histogram");
+    // LDS of the smallest configuration: 4 private copies (pitch nb + 1) and the edges
+    RGN_REQUIRE(static_cast<size_t>(nb + 1) * (4 * (sizeof(double) + sizeof(unsigned)) + sizeof(double)) <=
+                  150 * 1024,
+                RGN_ERR_UNSUPPORTED, "too many energy bins for the LDS histogram (at most 2741)");
     auto* plan = new rgn_hist_plan();
     plan->nb = nb;
     plan->log_bins = log_bins ? 1 : 0;
@@ -281,10 +292,14 @@ int rgn_energy_hist(const rgn_hist_plan* plan, const float* const d_u[3], size_t
     const size_t np = static_cast<size_t>(nb) * grid;
     double* partials = static_cast<double*>(scratch((sizeof(double) + sizeof(unsigned)) * np, 2));
     unsigned* cpart = reinterpret_cast<unsigned*>(partials + np);
-    const size_t smem = 4 * nb * (sizeof(double) + sizeof(unsigned)) + (nb + 1) * sizeof(double);
+    // lane groups per wave: as many private copies as fit in 48 KB of LDS
+    const size_t per_copy = static_cast<size_t>(nb + 1) * (sizeof(double) + sizeof(unsigned));
+    int qper = 4;
+    while (qper > 1 && 4 * qper * per_copy + (nb + 1) * sizeof(double) > 48 * 1024) qper /= 2;
+    const size_t smem = 4 * qper * per_copy + (nb + 1) * sizeof(double);
     hist_kernel<<<grid, kThreads, smem, st>>>(
       d_u[0], d_u[1], d_u[2], n, plan->d_edges.as<double>(), nb, plan->emin, plan->emax,
-      plan->approx_scale, plan->log_bins, fourvel ? 1 : 0, vec, cpart, partials);
+      plan->approx_scale, plan->log_bins, fourvel ? 1 : 0, vec, qper, cpart, partials);
     RGN_HIP(hipGetLastError());
     stage_mark(st, "hist_kernel");
     hist_combine_kernel<<<nb, kThreads, 0, st>>>(partials, cpart, grid, nb, d_hist,

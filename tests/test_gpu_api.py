@@ -253,3 +253,31 @@ def test_weights_extension(rg, orc):
     ex, f64 = oracle_pair(orc, u.astype(np.float32), b.astype(np.float32), bins.as_array(),
                           w=w.astype(np.float32))
     assert_spectrum(s, ex, f64, what="weights")
+
+
+def test_import_before_torch_uses_one_runtime():
+    """Importing ragnar_amd before torch leaves one HIP runtime in the process:
+    the drop-in module and torch device tensors both work afterwards."""
+    import subprocess
+    import sys
+    from pathlib import Path
+
+    code = (
+        "import numpy as np\n"
+        "import ragnar_amd\n"
+        "import torch\n"
+        "from ragnar_amd import ragnar as rg\n"
+        "x = torch.ones(8, device='cuda')\n"
+        "assert x.sum().item() == 8.0\n"
+        "p = rg.Particles_3D('e')\n"
+        "u = np.full(64, 3.0, np.float32)\n"
+        "b = np.full(64, 1.0, np.float32)\n"
+        "p.fromArrays({'U1': u, 'U2': u * 0, 'U3': u * 0, 'B1': b * 0, 'B2': b, 'B3': b * 0})\n"
+        "s = rg.SynchrotronSpectrum_3D(p, rg.Logbins(1e-2, 1e3, 32, 'mec2'), 1.0, 1.0, 1.0)\n"
+        "assert np.isfinite(s.as_array()).all() and s.as_array().max() > 0\n"
+        "print('ok')\n")
+    root = Path(__file__).resolve().parents[1]
+    r = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True,
+                       timeout=300, env={**__import__("os").environ, "RAGNAR_VERBOSE": "0",
+                                         "PYTHONPATH": str(root)})
+    assert r.returncode == 0 and "ok" in r.stdout, r.stdout + r.stderr

@@ -187,6 +187,22 @@ def test_hist_vs_oracle(orc, log_bins, fourvel):
     assert np.allclose(h[fin], hd[fin], rtol=1e-6, atol=0)
 
 
+@pytest.mark.parametrize("nb", [1, 900, 2741])
+def test_hist_bin_counts(orc, nb):
+    """Histogram plans whose LDS holds 16, 8 or 4 private copies per block (and a
+    single bin): counts bit-exact, sums 1e-6, against the oracle."""
+    u, _ = workloads.particles(100_000, seed=nb)
+    bins = orc.logspace(1e-2, 1e3, nb) if nb > 1 else np.array([1.0], np.float32)
+    plan = T.HistPlan(bins, True)
+    h, c = T.energy_hist(plan, dev(u), True)
+    h, c = h.cpu().numpy(), c.cpu().numpy()
+    ho, hd, co = orc.energy_hist(u, bins, True, True)
+    assert np.array_equal(c.astype(np.uint64), co)
+    assert int(c.sum()) == u.shape[1]
+    fin = np.isfinite(hd)
+    assert np.allclose(h[fin], hd[fin], rtol=1e-6, atol=0)
+
+
 def test_hist_exact_edges(orc):
     bins = orc.logspace(1e-2, 1e3, 200)
     plan = T.HistPlan(bins, True)
