@@ -84,9 +84,10 @@ void build_binned_tables(SyncPlan& plan, const std::vector<float>& xs, const std
     plan.kmin = 0;
     plan.NK = 0;
   }
-  // sub-cell lookup grid: jlut[q] = number of alphas below q / JQ
+  // sub-cell lookup grid: jlut[q] = number of alphas below q / JQ (about 8 grid
+  // cells per alpha, so the final scan is short)
   int JQ = 64;
-  while (JQ < 2 * plan.M && JQ < 8192) JQ *= 2;
+  while (JQ < 8 * plan.M && JQ < 8192) JQ *= 2;
   plan.JQ = JQ;
   std::vector<unsigned short> jlut(JQ);
   for (int q = 0; q < JQ; ++q) {
@@ -96,8 +97,16 @@ void build_binned_tables(SyncPlan& plan, const std::vector<float>& xs, const std
   plan.binned_ok = plan.M <= kBinnedMaxM && plan.NK <= kBinnedMaxNK;
 
   plan.alpha.reset(std::max<size_t>(1, alphas.size()) * sizeof(double));
+  plan.athr.reset(std::max<size_t>(1, alphas.size()) * sizeof(float));
   if (!alphas.empty()) {
     RGN_HIP(hipMemcpy(plan.alpha.ptr, alphas.data(), alphas.size() * sizeof(double), hipMemcpyHostToDevice));
+    std::vector<float> athr(alphas.size());
+    for (size_t i = 0; i < alphas.size(); ++i) {
+      float f = static_cast<float>(alphas[i]);
+      if (static_cast<double>(f) <= alphas[i]) f = std::nextafter(f, 2.0f);
+      athr[i] = f;
+    }
+    RGN_HIP(hipMemcpy(plan.athr.ptr, athr.data(), athr.size() * sizeof(float), hipMemcpyHostToDevice));
   }
   plan.jlut.reset(jlut.size() * sizeof(unsigned short));
   RGN_HIP(hipMemcpy(plan.jlut.ptr, jlut.data(), plan.jlut.bytes, hipMemcpyHostToDevice));
@@ -124,6 +133,7 @@ constexpr int kPrefThreads = 1024;         // 4 item groups x 256 sub-cells per 
 
 struct BinnedDev {
   const double* alpha;
+  const float* athr;
   const unsigned short* jlut;
   const int* rptr;
   const int* rlist;
@@ -136,7 +146,7 @@ struct BinnedDev {
 };
 
 BinnedDev binned_view(const SyncPlan& p) {
-  return BinnedDev{p.alpha.as<double>(), p.jlut.as<unsigned short>(), p.rptr.as<int>(), p.rlist.as<int>(),
+  return BinnedDev{p.alpha.as<double>(), p.athr.as<float>(), p.jlut.as<unsigned short>(), p.rptr.as<int>(), p.rlist.as<int>(),
                    p.binfo.as<BinInfo>(), p.seg.as<SegInfo>(), p.bine.as<float>(),
                    p.M, p.JQ, p.T, p.kmin, p.NK, p.nbins, emit_const(p), p.K};
 }
@@ -157,6 +167,15 @@ __device__ __forceinline__ int subcell(const double* __restrict__ alpha, const u
                                        int M, int JQ, double phi) {
   int j = jlut[min(static_cast<int>(phi * JQ), JQ - 1)];
   while (j < M && alpha[j] < phi) ++j;
+  return j;
+}
+
+// The same count for a float phi, with float thresholds: alpha[i] < phi
+// exactly when athr[i] <= phi.
+__device__ __forceinline__ int subcell_f(const float* __restrict__ athr, const unsigned short* __restrict__ jlut,
+                                         int M, int JQ, float phi) {
+  int j = jlut[min(static_cast<int>(phi * JQ), JQ - 1)];
+  while (j < M && athr[j] <= phi) ++j;
   return j;
 }
 
@@ -201,11 +220,10 @@ __global__ void __launch_bounds__(kThreads) fill_zero(unsigned* __restrict__ p, 
 }
 
 // Pass 1: per block of kChunk particles, sources sorted by bucket k.
-//   sortW/sortPhi[b * kChunk + i]: the block's sources in bucket order;
+//   sortWP[b * kChunk + i]: the block's sources (W, phi) in bucket order;
 //   runs[k * G + b]: run descriptor of bucket k in block b.
 __global__ void __launch_bounds__(kThreads) binned_bucket(ParticlesDev D, unsigned n, BinnedDev P,
-                                                          float* __restrict__ sortW, float* __restrict__ sortPhi,
-                                                          unsigned* __restrict__ runs) {
+                                                          float2* __restrict__ sortWP, unsigned* __restrict__ runs) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int NK = P.NK;
   unsigned* hist = reinterpret_cast<unsigned*>(smem);
@@ -256,10 +274,7 @@ __global__ void __launch_bounds__(kThreads) binned_bucket(ParticlesDev D, unsign
   }
   __syncthreads();
   const unsigned total = off[NK - 1] + hist[NK - 1];
-  for (unsigned i = threadIdx.x; i < total; i += kThreads) {
-    sortW[base + i] = sW[i];
-    sortPhi[base + i] = sP[i];
-  }
+  for (unsigned i = threadIdx.x; i < total; i += kThreads) sortWP[base + i] = make_float2(sW[i], sP[i]);
 }
 
 // Pass 2a: per bucket k, its non-empty runs in block order, compacted:
@@ -335,16 +350,15 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
                                                               const unsigned* __restrict__ cvoff,
                                                               const unsigned* __restrict__ cidx,
                                                               const unsigned* __restrict__ nrun,
-                                                              const float* __restrict__ sortW,
-                                                              const float* __restrict__ sortPhi,
+                                                              const float2* __restrict__ sortWP,
                                                               double* __restrict__ partial) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int kWaves = kSubThreads / 64;
   const int M1 = P.M + 1;
   double* m0 = reinterpret_cast<double*>(smem);
   double* m1 = m0 + M1;
-  double* alpha = m1 + M1;
-  unsigned* wro = reinterpret_cast<unsigned*>(alpha + P.M);  // kWin + 1
+  float* athr = reinterpret_cast<float*>(m1 + M1);
+  unsigned* wro = reinterpret_cast<unsigned*>(athr + P.M);   // kWin + 1
   unsigned* wlo = wro + kWin + 1;                              // kWin
   unsigned short* jlut = reinterpret_cast<unsigned short*>(wlo + kWin);
   __shared__ unsigned s_cnt[3];
@@ -355,7 +369,7 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
     m0[i] = 0.0;
     m1[i] = 0.0;
   }
-  for (int i = threadIdx.x; i < P.M; i += kSubThreads) alpha[i] = P.alpha[i];
+  for (int i = threadIdx.x; i < P.M; i += kSubThreads) athr[i] = P.athr[i];
   for (int i = threadIdx.x; i < P.JQ; i += kSubThreads) jlut[i] = P.jlut[i];
   __syncthreads();
   // Monotone searches as counts (all loads of a search in flight at once):
@@ -416,42 +430,59 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
         cur = lo;
       }
       // kBatch positions per lane at a time: resolve them through the run
-      // cursor, issue every load, then do the sub-cell lookups and atomics
-      for (unsigned v00 = pa; v00 < pb; v00 += 64 * kBatch) {
+      // cursor, issue every load, then do the sub-cell lookups and atomics.
+      // Full batches run without per-position guards.
+      auto advance = [&](unsigned v) {
+        cur += crs[cur + 1] <= v;  // the usual case: at most one run boundary
+        if (crs[cur + 1] <= v) {   // several short runs crossed
+          int lo = cur + 1, hi = nc - 1;
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (crs[mid] <= v) lo = mid; else hi = mid - 1;
+          }
+          cur = lo;
+        }
+      };
+      auto accumulate = [&](float2 wp) {
+        int j = jlut[min(static_cast<int>(wp.y * P.JQ), P.JQ - 1)];
+        j += j < P.M && athr[j] <= wp.y;
+        while (j < P.M && athr[j] <= wp.y) ++j;  // rare: a grid cell with several alphas
+        const double wd = wp.x;
+        atomicAdd(m0 + j, wd);
+        atomicAdd(m1 + j, wd * static_cast<double>(wp.y));
+      };
+      unsigned v00 = pa;
+      for (; v00 + 64 * kBatch <= pb; v00 += 64 * kBatch) {
+        unsigned idx[kBatch];
+#pragma unroll
+        for (int q = 0; q < kBatch; ++q) {
+          const unsigned v = v00 + q * 64 + lane;
+          advance(v);
+          idx[q] = cix[cur] + (v - crs[cur]);
+        }
+        float2 wp[kBatch];
+#pragma unroll
+        for (int q = 0; q < kBatch; ++q) wp[q] = sortWP[idx[q]];
+#pragma unroll
+        for (int q = 0; q < kBatch; ++q) accumulate(wp[q]);
+      }
+      if (v00 < pb) {  // the last, partial batch
         unsigned idx[kBatch];
 #pragma unroll
         for (int q = 0; q < kBatch; ++q) {
           const unsigned v = v00 + q * 64 + lane;
           idx[q] = ~0u;
           if (v < pb) {
-            if (crs[cur + 1] <= v) {
-              ++cur;
-              if (crs[cur + 1] <= v) {
-                int lo = cur + 1, hi = nc - 1;
-                while (lo < hi) {
-                  const int mid = (lo + hi + 1) >> 1;
-                  if (crs[mid] <= v) lo = mid; else hi = mid - 1;
-                }
-                cur = lo;
-              }
-            }
+            advance(v);
             idx[q] = cix[cur] + (v - crs[cur]);
           }
         }
-        float w[kBatch], ph[kBatch];
+        float2 wp[kBatch];
+#pragma unroll
+        for (int q = 0; q < kBatch; ++q) wp[q] = idx[q] != ~0u ? sortWP[idx[q]] : make_float2(0.0f, 0.0f);
 #pragma unroll
         for (int q = 0; q < kBatch; ++q) {
-          w[q] = idx[q] != ~0u ? sortW[idx[q]] : 0.0f;
-          ph[q] = idx[q] != ~0u ? sortPhi[idx[q]] : 0.0f;
-        }
-#pragma unroll
-        for (int q = 0; q < kBatch; ++q) {
-          if (idx[q] != ~0u) {
-            const double wd = w[q], phd = ph[q];
-            const int j = subcell(alpha, jlut, P.M, P.JQ, phd);
-            atomicAdd(m0 + j, wd);
-            atomicAdd(m1 + j, wd * phd);
-          }
+          if (idx[q] != ~0u) accumulate(wp[q]);
         }
       }
     }
@@ -670,7 +701,7 @@ static size_t bucket_smem(const SyncPlan& plan) {
 
 static size_t subhist_smem(const SyncPlan& plan) {
   const size_t M1 = plan.M + 1;
-  return sizeof(double) * (2 * M1 + plan.M) + sizeof(unsigned) * (2 * kWin + 1) +
+  return sizeof(double) * 2 * M1 + sizeof(float) * plan.M + sizeof(unsigned) * (2 * kWin + 1) +
          sizeof(unsigned short) * plan.JQ;
 }
 
@@ -686,8 +717,7 @@ void sync_forward_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, 
   const size_t M1 = plan.M + 1;
   // items: sum over k of ceil(totals_k / kItem) <= n / kItem + NK
   const size_t max_items = n / kItem + NK + 1;
-  float* sortW = static_cast<float*>(scratch(sizeof(float) * G * size_t{kChunk}, 6));
-  float* sortPhi = static_cast<float*>(scratch(sizeof(float) * G * size_t{kChunk}, 7));
+  float2* sortWP = static_cast<float2*>(scratch(sizeof(float2) * G * size_t{kChunk}, 6));
   unsigned* runs = static_cast<unsigned*>(scratch(sizeof(unsigned) * NK * G, 8));
   unsigned* cvoff = static_cast<unsigned*>(scratch(sizeof(unsigned) * NK * (G + 1), 10));
   unsigned* cidx = static_cast<unsigned*>(scratch(sizeof(unsigned) * NK * G, 9));
@@ -704,7 +734,7 @@ void sync_forward_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, 
     runs, NK * G);
   RGN_HIP(hipGetLastError());
   stage_mark(st, "fill_zero");
-  binned_bucket<<<G, kThreads, bucket_smem(plan), st>>>(D, static_cast<unsigned>(n), P, sortW, sortPhi, runs);
+  binned_bucket<<<G, kThreads, bucket_smem(plan), st>>>(D, static_cast<unsigned>(n), P, sortWP, runs);
   RGN_HIP(hipGetLastError());
   stage_mark(st, "binned_bucket");
   binned_scan_rows<<<plan.NK, kThreads, 0, st>>>(runs, G, cvoff, cidx, nrun, totals);
@@ -714,7 +744,7 @@ void sync_forward_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, 
   RGN_HIP(hipGetLastError());
   stage_mark(st, "binned_scan_items");
   binned_subhist<<<static_cast<unsigned>(max_items), kSubThreads, subhist_smem(plan), st>>>(
-    P, G, istart, totals, cvoff, cidx, nrun, sortW, sortPhi, partial);
+    P, G, istart, totals, cvoff, cidx, nrun, sortWP, partial);
   RGN_HIP(hipGetLastError());
   stage_mark(st, "binned_subhist");
   binned_prefix<<<plan.NK, kPrefThreads, sizeof(double) * 2 * M1, st>>>(plan.M, istart, partial, P0, P1);

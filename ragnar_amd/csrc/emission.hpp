@@ -38,7 +38,9 @@ struct SyncPlan {
   std::vector<double> A;           // A_e of live bins
   std::vector<char> live;          // bin e has e_e > 0
   DeviceBuffer alpha;              // double[M], sorted
-  DeviceBuffer jlut;               // uint16[JQ]: number of alphas below q / JQ
+  DeviceBuffer athr;               // float[M]: smallest float above alpha[i] (so, for a float
+                                   //   phi, alpha[i] < phi exactly when athr[i] <= phi)
+  DeviceBuffer jlut;               // uint16[JQ]: number of alphas below q / JQ (JQ >= 8 M)
   DeviceBuffer rptr;               // int[M + 1]: live bins with alpha rank r are
   DeviceBuffer rlist;              //   rlist[rptr[r] .. rptr[r + 1]) (ascending e)
   DeviceBuffer binfo;              // BinInfo[nbins]
