@@ -87,6 +87,23 @@ __device__ __forceinline__ double div_rn(double a, double b, double y) {
   return fma(fma(-q0, b, a), y, q0);
 }
 
+// sqrt(x) correctly rounded, for finite x with 2^-767 < x: the reciprocal
+// square root refinement the compiler expands sqrt into (hardware estimate, one
+// Goldschmidt step, two Newton corrections of the root), without the input
+// scaling it adds for tiny x and the special-value selects.  Equal to sqrt()
+// on this range (checked on the CPU with the estimate perturbed by 2^-22).
+__device__ __forceinline__ double sqrt_rn_normal(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = y * 0.5;
+  const double r = fma(-h, g, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  double d = fma(-g, g, x);
+  g = fma(d, h, g);
+  d = fma(-g, g, x);
+  return fma(d, h, g);
+}
+
 // Intermediate terms of OmegaSync_ChiR for one particle (the adjoint reuses them).
 struct OmegaTerms {
   double gamma, ig;        // gamma and RN(1 / gamma)
@@ -110,9 +127,10 @@ __device__ __forceinline__ OmegaTerms omega_terms(const RawParticle& r, const Em
   OmegaTerms o;
   const float ux = r.ux, uy = r.uy, uz = r.uz, ex = r.ex, ey = r.ey, ez = r.ez;
   const float bx = r.bx, by = r.by, bz = r.bz;
-  const double gamma = sqrt(1.0 + ux * ux + uy * uy + uz * uz);
+  const double g2 = 1.0 + ux * ux + uy * uy + uz * uz;  // >= 1, or inf / NaN
+  const bool fast = C.fast && g2 < 1e300;               // false for inf and NaN
+  const double gamma = fast ? sqrt_rn_normal(g2) : sqrt(g2);
   o.gamma = gamma;
-  const bool fast = C.fast && gamma < 1e300;  // false for inf and NaN
   if (fast) {
     const double ig = 1.0 / gamma;
     o.ig = ig;
@@ -130,7 +148,14 @@ __device__ __forceinline__ OmegaTerms omega_terms(const RawParticle& r, const Em
   o.vy = ey + (o.btz * bx - o.btx * bz);
   o.vz = ez + (o.btx * by - o.bty * bx);
   const double v_sqr = o.vx * o.vx + o.vy * o.vy + o.vz * o.vz;
-  o.rootQ = sqrt(v_sqr - o.bde * o.bde);
+  const double q = v_sqr - o.bde * o.bde;
+  if (fast) {
+    // from float inputs q is 0, negative (rounding: NaN, as sqrt), or above
+    // 1e-90 and finite
+    o.rootQ = q > 0.0 ? sqrt_rn_normal(q) : (q == 0.0 ? 0.0 : __builtin_nan(""));
+  } else {
+    o.rootQ = sqrt(q);
+  }
   if (fast) {
     o.chiR = static_cast<float>(div_rn(o.rootQ, static_cast<double>(C.B0), C.rB0));
     o.e_peak = static_cast<float>(div_rn(C.e_at * gamma * gamma * o.chiR, C.Dg, C.rDg));
