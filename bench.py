@@ -308,9 +308,9 @@ def roofline(prof, n, nb, plan):
     forward_kernel (direct): 6 FP32 VALU flop per particle x bin eval (the
       inner loop: two FMAs forming u, one table-index conversion, the FMA of
       the interpolant and the accumulating FMA); bound: FP32 VALU.
-    binned_bucket: reads u and B (24 B/particle), writes each contributing
-      source's (W, phi) (8 B/particle) and its block's per-bucket run table
-      (2 x NK x blocks x 4 B); bound: HBM.
+    binned_bucket: reads u and B (24 B/particle) and writes each contributing
+      source's (W, phi) (8 B/particle); its run descriptors (4 B per non-empty
+      run, ~36 sources per run here) are left out; bound: HBM.
     binned_subhist: reads the sorted (W, phi) (8 B/particle) and writes its
       per-item moments (2 x (M + 1) x 8 B per item); bound: HBM.
     """
@@ -324,7 +324,7 @@ def roofline(prof, n, nb, plan):
                 "kernel": name, "kernel_ms": ms, "flop_per_eval": FLOP_PER_EVAL_FWD}
     blocks = -(-n // 4096)
     if name == "binned_bucket":
-        nbytes = 32 * n + 2 * 4 * blocks * info["NK"]
+        nbytes = 32 * n
     elif name == "binned_subhist":
         items = n // 16384 + info["NK"]
         nbytes = 8 * n + 16 * (info["M"] + 1) * items

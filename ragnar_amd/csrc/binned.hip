@@ -222,6 +222,7 @@ __global__ void __launch_bounds__(kThreads) fill_zero(unsigned* __restrict__ p, 
 // Pass 1: per block of kChunk particles, sources sorted by bucket k.
 //   sortWP[b * kChunk + i]: the block's sources (W, phi) in bucket order;
 //   runs[k * G + b]: run descriptor of bucket k in block b.
+template <bool HAS_E>
 __global__ void __launch_bounds__(kThreads) binned_bucket(ParticlesDev D, unsigned n, BinnedDev P,
                                                           float2* __restrict__ sortWP, unsigned* __restrict__ runs) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -251,7 +252,7 @@ __global__ void __launch_bounds__(kThreads) binned_bucket(ParticlesDev D, unsign
     rk[i] = 0u;
     float W;
     double B;
-    if (p < n && particle_wb(omega_terms(cur, P.em), cur, P, W, B)) {
+    if (p < n && particle_wb(omega_terms<HAS_E>(cur, P.em), cur, P, W, B)) {
       const double fl = floor(B);
       kk[i] = static_cast<int>(fl) - P.kmin;
       Wr[i] = W;
@@ -282,14 +283,16 @@ __global__ void __launch_bounds__(kThreads) binned_bucket(ParticlesDev D, unsign
 //                           (cvoff[k * (G + 1) + nrun[k]] = totals[k]);
 //   cidx[k * G + i]:        physical index of the run's first source.
 // One exclusive scan over blocks of (run length, non-empty flag) packed in 64 bits.
-__global__ void __launch_bounds__(kThreads) binned_scan_rows(const unsigned* __restrict__ runs, int G,
+// The descriptors read are zeroed again, so the table needs no clearing pass
+// before the next call.
+__global__ void __launch_bounds__(kThreads) binned_scan_rows(unsigned* __restrict__ runs, int G,
                                                              unsigned* __restrict__ cvoff,
                                                              unsigned* __restrict__ cidx,
                                                              unsigned* __restrict__ nrun,
                                                              unsigned* __restrict__ totals) {
   __shared__ unsigned long long part[kThreads];
   const int k = blockIdx.x;
-  const unsigned* row = runs + static_cast<size_t>(k) * G;
+  unsigned* row = runs + static_cast<size_t>(k) * G;
   unsigned* vo = cvoff + static_cast<size_t>(k) * (G + 1);
   unsigned* ci = cidx + static_cast<size_t>(k) * G;
   const int per = (G + kThreads - 1) / kThreads;
@@ -316,6 +319,7 @@ __global__ void __launch_bounds__(kThreads) binned_scan_rows(const unsigned* __r
       vo[c] = static_cast<unsigned>(acc);
       ci[c] = static_cast<unsigned>(i) * kChunk + run_start(r);
       acc += len + (1ull << 32);
+      row[i] = 0u;  // leave the run table zeroed for the next call
     }
   }
   if (threadIdx.x == kThreads - 1) {
@@ -646,6 +650,7 @@ struct GradPtrs {
   float* ge[3];
 };
 
+template <bool HAS_E>
 __global__ void __launch_bounds__(kThreads) binned_adj_gather(ParticlesDev D, size_t n, BinnedDev P,
                                                               const double2* __restrict__ U, GradPtrs G) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -665,7 +670,7 @@ __global__ void __launch_bounds__(kThreads) binned_adj_gather(ParticlesDev D, si
     float W;
     double B;
     ParticleGrads g{};
-    const OmegaTerms o = omega_terms_approx(cur, P.em);
+    const OmegaTerms o = omega_terms_approx<HAS_E>(cur, P.em);
     if (particle_wb(o, cur, P, W, B)) {
       const double fl = floor(B);
       const double phi = B - fl;
@@ -718,7 +723,8 @@ void sync_forward_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, 
   // items: sum over k of ceil(totals_k / kItem) <= n / kItem + NK
   const size_t max_items = n / kItem + NK + 1;
   float2* sortWP = static_cast<float2*>(scratch(sizeof(float2) * G * size_t{kChunk}, 6));
-  unsigned* runs = static_cast<unsigned*>(scratch(sizeof(unsigned) * NK * G, 8));
+  bool fresh = false;
+  unsigned* runs = static_cast<unsigned*>(scratch(sizeof(unsigned) * NK * G, 8, &fresh));
   unsigned* cvoff = static_cast<unsigned*>(scratch(sizeof(unsigned) * NK * (G + 1), 10));
   unsigned* cidx = static_cast<unsigned*>(scratch(sizeof(unsigned) * NK * G, 9));
   unsigned* small = static_cast<unsigned*>(scratch(sizeof(unsigned) * (3 * (NK + 1) + 64), 11));
@@ -730,15 +736,31 @@ void sync_forward_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, 
   double* P0 = pref;
   double* P1 = pref + M1 * NK;
 
-  fill_zero<<<static_cast<unsigned>(std::min<size_t>(ceil_div(NK * G, kThreads), 4096)), kThreads, 0, st>>>(
-    runs, NK * G);
-  RGN_HIP(hipGetLastError());
-  stage_mark(st, "fill_zero");
-  binned_bucket<<<G, kThreads, bucket_smem(plan), st>>>(D, static_cast<unsigned>(n), P, sortWP, runs);
+  // Leading entries of the run table known to be zero: every call's scan pass
+  // re-zeroes the descriptors its bucket pass wrote, so a fill is needed only
+  // for a new buffer or a larger table (reset before the bucket pass, so that a
+  // call interrupted in between leaves the next one to fill).
+  static size_t runs_clean = 0;
+  if (fresh) runs_clean = 0;
+  if (NK * G > runs_clean) {
+    fill_zero<<<static_cast<unsigned>(std::min<size_t>(ceil_div(NK * G, kThreads), 4096)), kThreads, 0, st>>>(
+      runs, NK * G);
+    RGN_HIP(hipGetLastError());
+    stage_mark(st, "fill_zero");
+  }
+  const size_t clean_after = std::max(runs_clean, NK * G);
+  runs_clean = 0;
+  const bool has_e = D.e[0] || D.e[1] || D.e[2];
+  if (has_e) {
+    binned_bucket<true><<<G, kThreads, bucket_smem(plan), st>>>(D, static_cast<unsigned>(n), P, sortWP, runs);
+  } else {
+    binned_bucket<false><<<G, kThreads, bucket_smem(plan), st>>>(D, static_cast<unsigned>(n), P, sortWP, runs);
+  }
   RGN_HIP(hipGetLastError());
   stage_mark(st, "binned_bucket");
   binned_scan_rows<<<plan.NK, kThreads, 0, st>>>(runs, G, cvoff, cidx, nrun, totals);
   RGN_HIP(hipGetLastError());
+  runs_clean = clean_after;
   stage_mark(st, "binned_scan_rows");
   binned_scan_items<<<1, kThreads, 0, st>>>(totals, plan.NK, istart);
   RGN_HIP(hipGetLastError());
@@ -779,7 +801,11 @@ void sync_adjoint_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, 
   const int grid = static_cast<int>(std::max<size_t>(1, std::min<size_t>(ceil_div(n, kThreads),
                                                                             static_cast<size_t>(num_cus()) * 8)));
   const size_t smem = sizeof(double) * std::max(plan.M, 1) + sizeof(unsigned short) * plan.JQ;
-  binned_adj_gather<<<grid, kThreads, smem, st>>>(D, n, Pg, U, G);
+  if (D.e[0] || D.e[1] || D.e[2]) {
+    binned_adj_gather<true><<<grid, kThreads, smem, st>>>(D, n, Pg, U, G);
+  } else {
+    binned_adj_gather<false><<<grid, kThreads, smem, st>>>(D, n, Pg, U, G);
+  }
   RGN_HIP(hipGetLastError());
   stage_mark(st, "binned_adj_gather");
 }

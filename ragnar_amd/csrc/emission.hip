@@ -298,13 +298,15 @@ __global__ void __launch_bounds__(kThreads) adjoint_kernel(ParticlesDev D, size_
   }
 }
 
-// Per-particle (e_peak, chiR, W, V_hi, V_lo) for diagnostics.
+// Per-particle (e_peak, chiR, W, V_hi, V_lo) for diagnostics; HAS_E as the
+// factorised kernels instantiate it (no electric field: the shorter form).
+template <bool HAS_E>
 __global__ void __launch_bounds__(kThreads) particle_terms_kernel(ParticlesDev D, size_t n,
                                                                   PlanDev P, float* __restrict__ out) {
   const size_t p = blockIdx.x * static_cast<size_t>(kThreads) + threadIdx.x;
   if (p >= n) return;
   const RawParticle r = load_particle(D, p);
-  const OmegaTerms o = omega_terms(r, P.em);
+  const OmegaTerms o = omega_terms<HAS_E>(r, P.em);
   const Source s = particle_source(r, P);
   out[5 * p + 0] = o.e_peak;
   out[5 * p + 1] = o.chiR;
@@ -700,8 +702,12 @@ int rgn_sync_particle_terms(const rgn_sync_plan* plan, const float* const d_u[3]
     RGN_REQUIRE(plan != nullptr, RGN_ERR_INVALID, "null plan");
     const ParticlesDev D = make_particles_checked(d_u, d_e, d_b, d_w, n);
     if (!n) return;
-    particle_terms_kernel<<<static_cast<unsigned>(ceil_div(n, kThreads)), kThreads, 0,
-                            as_stream(stream)>>>(D, n, device_view(plan->p), d_out);
+    const unsigned grid = static_cast<unsigned>(ceil_div(n, kThreads));
+    if (D.e[0] || D.e[1] || D.e[2]) {
+      particle_terms_kernel<true><<<grid, kThreads, 0, as_stream(stream)>>>(D, n, device_view(plan->p), d_out);
+    } else {
+      particle_terms_kernel<false><<<grid, kThreads, 0, as_stream(stream)>>>(D, n, device_view(plan->p), d_out);
+    }
     RGN_HIP(hipGetLastError());
   });
 }

@@ -123,6 +123,10 @@ struct OmegaTerms {
 // returns the same doubles as the division operator; a particle with a
 // non-finite gamma, or a plan whose divisors are not normal, takes the
 // division operator itself.
+// HAS_E = false: the particle store has no electric field.  Then beta.E is
+// exactly 0 and V = beta x B, the reference's values up to the sign of zero
+// terms that only enter squared.
+template <bool HAS_E = true>
 __device__ __forceinline__ OmegaTerms omega_terms(const RawParticle& r, const EmitConst& C) {
   OmegaTerms o;
   const float ux = r.ux, uy = r.uy, uz = r.uz, ex = r.ex, ey = r.ey, ez = r.ez;
@@ -143,12 +147,19 @@ __device__ __forceinline__ OmegaTerms omega_terms(const RawParticle& r, const Em
     o.bty = uy / gamma;
     o.btz = uz / gamma;
   }
-  o.bde = o.btx * ex + o.bty * ey + o.btz * ez;
-  o.vx = ex + (o.bty * bz - o.btz * by);
-  o.vy = ey + (o.btz * bx - o.btx * bz);
-  o.vz = ez + (o.btx * by - o.bty * bx);
+  if (HAS_E) {
+    o.bde = o.btx * ex + o.bty * ey + o.btz * ez;
+    o.vx = ex + (o.bty * bz - o.btz * by);
+    o.vy = ey + (o.btz * bx - o.btx * bz);
+    o.vz = ez + (o.btx * by - o.bty * bx);
+  } else {
+    o.bde = 0.0;
+    o.vx = o.bty * bz - o.btz * by;
+    o.vy = o.btz * bx - o.btx * bz;
+    o.vz = o.btx * by - o.bty * bx;
+  }
   const double v_sqr = o.vx * o.vx + o.vy * o.vy + o.vz * o.vz;
-  const double q = v_sqr - o.bde * o.bde;
+  const double q = HAS_E ? v_sqr - o.bde * o.bde : v_sqr;
   if (fast) {
     // from float inputs q is 0, negative (rounding: NaN, as sqrt), or above
     // 1e-90 and finite
@@ -190,6 +201,7 @@ __device__ __forceinline__ double sqrt_nr(double x) {
 // gradients (which need no correct rounding): reciprocal products instead of
 // divisions, refined square roots.  A particle's e_peak may differ from the
 // reference's in the last float bit; the forward kernels use omega_terms.
+template <bool HAS_E = true>
 __device__ __forceinline__ OmegaTerms omega_terms_approx(const RawParticle& r, const EmitConst& C) {
   OmegaTerms o;
   const float ux = r.ux, uy = r.uy, uz = r.uz, ex = r.ex, ey = r.ey, ez = r.ez;
@@ -201,12 +213,19 @@ __device__ __forceinline__ OmegaTerms omega_terms_approx(const RawParticle& r, c
   o.btx = ux * ig;
   o.bty = uy * ig;
   o.btz = uz * ig;
-  o.bde = o.btx * ex + o.bty * ey + o.btz * ez;
-  o.vx = ex + (o.bty * bz - o.btz * by);
-  o.vy = ey + (o.btz * bx - o.btx * bz);
-  o.vz = ez + (o.btx * by - o.bty * bx);
+  if (HAS_E) {
+    o.bde = o.btx * ex + o.bty * ey + o.btz * ez;
+    o.vx = ex + (o.bty * bz - o.btz * by);
+    o.vy = ey + (o.btz * bx - o.btx * bz);
+    o.vz = ez + (o.btx * by - o.bty * bx);
+  } else {
+    o.bde = 0.0;
+    o.vx = o.bty * bz - o.btz * by;
+    o.vy = o.btz * bx - o.btx * bz;
+    o.vz = o.btx * by - o.bty * bx;
+  }
   const double v_sqr = o.vx * o.vx + o.vy * o.vy + o.vz * o.vz;
-  o.rootQ = sqrt_nr(v_sqr - o.bde * o.bde);
+  o.rootQ = sqrt_nr(HAS_E ? v_sqr - o.bde * o.bde : v_sqr);
   o.chiR = static_cast<float>(o.rootQ * C.rB0);
   o.e_peak = static_cast<float>(C.e_at * gamma * gamma * o.chiR * C.rDg);
   return o;

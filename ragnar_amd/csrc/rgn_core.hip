@@ -118,8 +118,11 @@ void stage_mark(hipStream_t st, const char* name) {
   ++s->n;
 }
 
-void* scratch(size_t bytes, int slot) {
+void* scratch(size_t bytes, int slot) { return scratch(bytes, slot, nullptr); }
+
+void* scratch(size_t bytes, int slot, bool* fresh) {
   std::lock_guard<std::mutex> lk(g_scratch_mutex);
+  if (fresh) *fresh = false;
   if (static_cast<size_t>(slot) >= g_scratch.size()) g_scratch.resize(slot + 1);
   Scratch& s = g_scratch[slot];
   if (s.bytes < bytes) {
@@ -133,6 +136,7 @@ void* scratch(size_t bytes, int slot) {
     const size_t grow = bytes + bytes / 4 + 4096;
     RGN_HIP(hipMalloc(&s.ptr, grow));
     s.bytes = grow;
+    if (fresh) *fresh = true;
   }
   return s.ptr;
 }

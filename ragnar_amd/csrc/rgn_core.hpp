@@ -74,6 +74,9 @@ struct DeviceBuffer {
 // demand; never shrinks).  Calls that overlap on different streams must not
 // share it; the library serialises scratch users through `scratch_lock`.
 void* scratch(size_t bytes, int slot);
+// The same; *fresh is set when the slot was (re)allocated by this call, so its
+// previous contents are gone.
+void* scratch(size_t bytes, int slot, bool* fresh);
 
 // Number of compute units of the current device.
 int num_cus();

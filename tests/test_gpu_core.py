@@ -65,13 +65,15 @@ def test_ffunc_integrand(orc):
         assert np.float32(rg.Ffunc_integrand(x)) == orc.ffunc_integrand(x), x
 
 
-@pytest.mark.parametrize("seed,consts", [(1, (1.3, 2.0, 0.7)), (2, (1.3, 2.0, 0.7)),
-                                         (3, (1.0, 1.0, 1.0)), (4, (0.37, 3.1e-3, 41.0))])
-def test_particle_terms_bit_exact(orc, table, seed, consts):
+@pytest.mark.parametrize("seed,consts,with_e", [(1, (1.3, 2.0, 0.7), True), (2, (1.3, 2.0, 0.7), True),
+                                                (3, (1.0, 1.0, 1.0), True), (4, (0.37, 3.1e-3, 41.0), True),
+                                                (5, (1.0, 1.0, 1.0), False), (6, (0.37, 3.1e-3, 41.0), False)])
+def test_particle_terms_bit_exact(orc, table, seed, consts, with_e):
     """e_peak and chiR of every particle equal the reference arithmetic bit for
     bit, over momenta from 1e-30 to 1e18 (including exact zeros and momenta
     whose squares overflow float), fields of both signs and several
-    (B0, g_syn, e_at)."""
+    (B0, g_syn, e_at); with an electric field and without one (the kernels'
+    shorter no-field form against the reference with E = 0)."""
     n = 4000
     rng = np.random.default_rng(seed)
     u, b = workloads.particles(n, seed=seed)
@@ -82,8 +84,10 @@ def test_particle_terms_bit_exact(orc, table, seed, consts):
     e = rng.normal(0, 0.05, (3, n)).astype(np.float32)
     e[:, : n // 4] *= -3.0
     bins = orc.logspace(1e-2, 1e7, 256)
+    if not with_e:
+        e[:] = 0.0
     plan = T.SyncPlan(table, bins, *consts)
-    terms = T.particle_terms(plan, dev(u), dev(b), None, dev(e)).cpu().numpy()
+    terms = T.particle_terms(plan, dev(u), dev(b), None, dev(e) if with_e else None).cpu().numpy()
     for p in range(n):
         ep, chi = orc.omega_sync(u[:, p], e[:, p], b[:, p], *consts)
         assert np.array_equal(terms[p, 0], ep, equal_nan=True), (p, terms[p, 0], ep)
