@@ -179,6 +179,32 @@ __device__ __forceinline__ int subcell_f(const float* __restrict__ athr, const u
   return j;
 }
 
+// Exclusive prefix of per-thread values over an NT-thread block: a shuffle
+// scan inside each wave, then the wave totals (wsum: NT / 64 words of LDS),
+// added in a fixed order by every thread.  Sets total; every thread calls it.
+template <class T, int NT>
+__device__ __forceinline__ T block_prefix(T v, T& total, T* wsum) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  T x = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const T t = __shfl_up(x, d);
+    if (lane >= d) x += t;
+  }
+  if (lane == 63) wsum[wave] = x;
+  __syncthreads();
+  T before = T(0), tot = T(0);
+#pragma unroll
+  for (int w = 0; w < NT / 64; ++w) {
+    const T sw = wsum[w];
+    if (w < wave) before += sw;
+    tot += sw;
+  }
+  total = tot;
+  __syncthreads();  // wsum may be reused
+  return before + (x - v);
+}
+
 // Block-wide exclusive scan of a[0..N) into o[0..N) (o may alias a); returns
 // the total.  part: NT words of LDS.  Every thread of the NT-thread block calls it.
 template <class T, int NT = kThreads, class PA, class PO>
@@ -187,16 +213,8 @@ __device__ T block_scan_exclusive(PA a, PO o, int N, T* part) {
   const int i0 = min(N, static_cast<int>(threadIdx.x) * per), i1 = min(N, i0 + per);
   T s = 0;
   for (int i = i0; i < i1; ++i) s += a[i];
-  part[threadIdx.x] = s;
-  __syncthreads();
-  for (int d = 1; d < NT; d <<= 1) {
-    const T v = static_cast<int>(threadIdx.x) >= d ? part[threadIdx.x - d] : T(0);
-    __syncthreads();
-    part[threadIdx.x] += v;
-    __syncthreads();
-  }
-  T acc = threadIdx.x ? part[threadIdx.x - 1] : T(0);
-  const T total = part[NT - 1];
+  T total;
+  T acc = block_prefix<T, NT>(s, total, part);
   for (int i = i0; i < i1; ++i) {
     const T v = a[i];
     o[i] = acc;
@@ -207,8 +225,9 @@ __device__ T block_scan_exclusive(PA a, PO o, int N, T* part) {
 }
 
 // Run descriptor of bucket k in bucket block b: start << 16 | length (both at
-// most kChunk = 4096).  The table is zero-filled first and only non-empty runs
-// are written (each scattered 4-byte store costs a 32-byte write-back).
+// most kChunk = 4096).  Only non-empty runs are written (each scattered 4-byte
+// store costs a 32-byte write-back); the scan pass zeroes what it reads, so the
+// table stays clear between calls and fill_zero runs only on a new buffer.
 __device__ __forceinline__ unsigned run_len(unsigned r) { return r & 0xffffu; }
 __device__ __forceinline__ unsigned run_start(unsigned r) { return r >> 16; }
 
@@ -290,29 +309,33 @@ __global__ void __launch_bounds__(kThreads) binned_scan_rows(unsigned* __restric
                                                              unsigned* __restrict__ cidx,
                                                              unsigned* __restrict__ nrun,
                                                              unsigned* __restrict__ totals) {
-  __shared__ unsigned long long part[kThreads];
+  __shared__ unsigned long long part[kThreads / 64];
   const int k = blockIdx.x;
   unsigned* row = runs + static_cast<size_t>(k) * G;
   unsigned* vo = cvoff + static_cast<size_t>(k) * (G + 1);
   unsigned* ci = cidx + static_cast<size_t>(k) * G;
   const int per = (G + kThreads - 1) / kThreads;
   const int i0 = min(G, static_cast<int>(threadIdx.x) * per), i1 = min(G, i0 + per);
+  // up to kRowCache descriptors per thread stay in registers between the passes
+  constexpr int kRowCache = 32;
+  unsigned cache[kRowCache];
   unsigned long long acc = 0;
-  for (int i = i0; i < i1; ++i) {
-    const unsigned len = run_len(row[i]);
-    acc += len + (len ? (1ull << 32) : 0ull);
+  if (per <= kRowCache) {
+#pragma unroll
+    for (int q = 0; q < kRowCache; ++q) {
+      cache[q] = i0 + q < i1 ? row[i0 + q] : 0u;
+      const unsigned len = run_len(cache[q]);
+      acc += len + (len ? (1ull << 32) : 0ull);
+    }
+  } else {
+    for (int i = i0; i < i1; ++i) {
+      const unsigned len = run_len(row[i]);
+      acc += len + (len ? (1ull << 32) : 0ull);
+    }
   }
-  part[threadIdx.x] = acc;
-  __syncthreads();
-  for (int d = 1; d < kThreads; d <<= 1) {
-    const unsigned long long v = static_cast<int>(threadIdx.x) >= d ? part[threadIdx.x - d] : 0ull;
-    __syncthreads();
-    part[threadIdx.x] += v;
-    __syncthreads();
-  }
-  acc = threadIdx.x ? part[threadIdx.x - 1] : 0ull;
-  for (int i = i0; i < i1; ++i) {
-    const unsigned r = row[i];
+  unsigned long long all;
+  acc = block_prefix<unsigned long long, kThreads>(acc, all, part);
+  auto emit = [&](int i, unsigned r) {
     const unsigned len = run_len(r);
     if (len) {
       const unsigned c = static_cast<unsigned>(acc >> 32);
@@ -321,10 +344,18 @@ __global__ void __launch_bounds__(kThreads) binned_scan_rows(unsigned* __restric
       acc += len + (1ull << 32);
       row[i] = 0u;  // leave the run table zeroed for the next call
     }
+  };
+  if (per <= kRowCache) {
+#pragma unroll
+    for (int q = 0; q < kRowCache; ++q) {
+      if (i0 + q < i1) emit(i0 + q, cache[q]);
+    }
+  } else {
+    for (int i = i0; i < i1; ++i) emit(i, row[i]);
   }
   if (threadIdx.x == kThreads - 1) {
-    const unsigned nr = static_cast<unsigned>(part[kThreads - 1] >> 32);
-    const unsigned total = static_cast<unsigned>(part[kThreads - 1]);
+    const unsigned nr = static_cast<unsigned>(all >> 32);
+    const unsigned total = static_cast<unsigned>(all);
     vo[nr] = total;
     nrun[k] = nr;
     totals[k] = total;
@@ -801,11 +832,9 @@ void sync_adjoint_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, 
   const int grid = static_cast<int>(std::max<size_t>(1, std::min<size_t>(ceil_div(n, kThreads),
                                                                             static_cast<size_t>(num_cus()) * 8)));
   const size_t smem = sizeof(double) * std::max(plan.M, 1) + sizeof(unsigned short) * plan.JQ;
-  if (D.e[0] || D.e[1] || D.e[2]) {
-    binned_adj_gather<true><<<grid, kThreads, smem, st>>>(D, n, Pg, U, G);
-  } else {
-    binned_adj_gather<false><<<grid, kThreads, smem, st>>>(D, n, Pg, U, G);
-  }
+  // the general form also for stores without E: the shorter one measured ~4%
+  // slower here (more resident waves, same memory traffic)
+  binned_adj_gather<true><<<grid, kThreads, smem, st>>>(D, n, Pg, U, G);
   RGN_HIP(hipGetLastError());
   stage_mark(st, "binned_adj_gather");
 }

@@ -288,15 +288,18 @@ int rgn_energy_hist(const rgn_hist_plan* plan, const float* const d_u[3], size_t
     const int vec = aligned16(d_u[0]) && aligned16(d_u[1]) && aligned16(d_u[2]);
     const size_t per_thread = vec ? 8 : 1;
     const size_t tiles = ceil_div(std::max<size_t>(n, 1), kThreads * per_thread);
-    const int grid = static_cast<int>(std::min<size_t>(tiles, static_cast<size_t>(num_cus()) * 8));
-    const size_t np = static_cast<size_t>(nb) * grid;
-    double* partials = static_cast<double*>(scratch((sizeof(double) + sizeof(unsigned)) * np, 2));
-    unsigned* cpart = reinterpret_cast<unsigned*>(partials + np);
     // lane groups per wave: as many private copies as fit in 48 KB of LDS
     const size_t per_copy = static_cast<size_t>(nb + 1) * (sizeof(double) + sizeof(unsigned));
     int qper = 4;
     while (qper > 1 && 4 * qper * per_copy + (nb + 1) * sizeof(double) > 48 * 1024) qper /= 2;
     const size_t smem = 4 * qper * per_copy + (nb + 1) * sizeof(double);
+    // one round of blocks: as many per CU as the LDS holds (at most 8), so the
+    // per-block clearing and reduction of the private copies is paid once
+    const size_t per_cu = std::max<size_t>(1, std::min<size_t>(8, (160 * 1024) / smem));
+    const int grid = static_cast<int>(std::min<size_t>(tiles, static_cast<size_t>(num_cus()) * per_cu));
+    const size_t np = static_cast<size_t>(nb) * grid;
+    double* partials = static_cast<double*>(scratch((sizeof(double) + sizeof(unsigned)) * np, 2));
+    unsigned* cpart = reinterpret_cast<unsigned*>(partials + np);
     hist_kernel<<<grid, kThreads, smem, st>>>(
       d_u[0], d_u[1], d_u[2], n, plan->d_edges.as<double>(), nb, plan->emin, plan->emax,
       plan->approx_scale, plan->log_bins, fourvel ? 1 : 0, vec, qper, cpart, partials);
