@@ -116,7 +116,7 @@ int rgn_sync_adjoint(const rgn_sync_plan* plan, const float* const d_u[3],
  * moments (forward), or read one precomputed entry each (adjoint); cost is
  * ~independent of the bin count.  RGN_METHOD_AUTO picks the cheaper one from a
  * measured cost model (forward: binned from ~1.5e6 particles at 256 bins; env
- * RAGNAR_SYNC_METHOD=direct|binned overrides).  Binned needs fewer than 2^31
+ * RAGNAR_SYNC_METHOD=direct|binned overrides).  Binned needs fewer than 2^29
  * particles, at most 1024 distinct fractional bin offsets and a bin range of
  * at most 3072 table units (RGN_ERR_UNSUPPORTED otherwise; AUTO then runs
  * direct).  Same results to FP32/FP64 rounding (no reference counterpart). */

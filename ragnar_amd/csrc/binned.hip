@@ -17,12 +17,13 @@
 // Forward (a scatter), no global atomics (device-scope FP64 atomics run at
 // ~14 G/s on MI355X):
 //   bucket    each block of 4096 particles computes (k, W, phi), counting-sorts
-//             its sources by k in LDS and writes the sorted chunk coalesced;
-//             per (bucket, block) run lengths and offsets go to [k][block] tables;
-//   scan      per bucket, the exclusive scan of its run lengths over blocks;
-//             then work items of up to kItem sources per bucket;
-//   subhist   each item gathers its runs (a virtual range of the bucket) and
-//             reduces them into M + 1 sub-cells in LDS (ds_add_f64);
+//             its sources by k in LDS and writes the sorted (W, phi) chunk
+//             coalesced, plus one run descriptor per non-empty bucket;
+//   scan      per bucket, its non-empty runs compacted in block order with
+//             their offsets within the bucket; then work items of up to kItem
+//             sources per bucket;
+//   subhist   each item walks its runs and reduces the sources into M + 1
+//             sub-cells in LDS (ds_add_f64);
 //   prefix    per bucket: the items in order, then prefix sums over sub-cells;
 //   evaluate  one block per photon bin, fixed-order reduction over k.
 // Adjoint (a gather): dL/dS is folded into a table U[k][j] = (sum_e g'_e C,
@@ -260,11 +261,11 @@ __global__ void __launch_bounds__(kThreads) binned_bucket(ParticlesDev D, unsign
   unsigned rk[kPer];
   unsigned p = base + threadIdx.x;
   RawParticle raw{};
-  if (p < n) raw = load_particle(D, p);
+  if (p < n) raw = load_particle32(D, p);
 #pragma unroll
   for (int i = 0; i < kPer; ++i, p += kThreads) {
     const RawParticle cur = raw;
-    if (i + 1 < kPer && p + kThreads < n) raw = load_particle(D, p + kThreads);
+    if (i + 1 < kPer && p + kThreads < n) raw = load_particle32(D, p + kThreads);
     kk[i] = -1;
     Wr[i] = 0.0f;
     ph[i] = 0.0f;
@@ -294,7 +295,7 @@ __global__ void __launch_bounds__(kThreads) binned_bucket(ParticlesDev D, unsign
   }
   __syncthreads();
   const unsigned total = off[NK - 1] + hist[NK - 1];
-  for (unsigned i = threadIdx.x; i < total; i += kThreads) sortWP[base + i] = make_float2(sW[i], sP[i]);
+  for (unsigned i = threadIdx.x; i < total; i += kThreads) st32(sortWP, base + i, make_float2(sW[i], sP[i]));
 }
 
 // Pass 2a: per bucket k, its non-empty runs in block order, compacted:
@@ -497,7 +498,7 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
         }
         float2 wp[kBatch];
 #pragma unroll
-        for (int q = 0; q < kBatch; ++q) wp[q] = sortWP[idx[q]];
+        for (int q = 0; q < kBatch; ++q) wp[q] = ld32(sortWP, idx[q]);
 #pragma unroll
         for (int q = 0; q < kBatch; ++q) accumulate(wp[q]);
       }
@@ -514,7 +515,7 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
         }
         float2 wp[kBatch];
 #pragma unroll
-        for (int q = 0; q < kBatch; ++q) wp[q] = idx[q] != ~0u ? sortWP[idx[q]] : make_float2(0.0f, 0.0f);
+        for (int q = 0; q < kBatch; ++q) wp[q] = idx[q] != ~0u ? ld32(sortWP, idx[q]) : make_float2(0.0f, 0.0f);
 #pragma unroll
         for (int q = 0; q < kBatch; ++q) {
           if (idx[q] != ~0u) accumulate(wp[q]);
@@ -691,13 +692,15 @@ __global__ void __launch_bounds__(kThreads) binned_adj_gather(ParticlesDev D, si
   for (int i = threadIdx.x; i < P.JQ; i += kThreads) jlut[i] = P.jlut[i];
   __syncthreads();
   const double dlnx = P.K / 0.69314718055994530942;  // dt / dln x
-  const size_t stride = static_cast<size_t>(gridDim.x) * kThreads;
-  size_t p = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x;
+  // n < 2^29 (the factorised form's limit): 32-bit indices and byte offsets
+  const unsigned nn = static_cast<unsigned>(n);
+  const unsigned stride = gridDim.x * kThreads;
+  unsigned p = blockIdx.x * kThreads + threadIdx.x;
   RawParticle raw{};
-  if (p < n) raw = load_particle(D, p);
-  for (; p < n; p += stride) {
+  if (p < nn) raw = load_particle32(D, p);
+  for (; p < nn; p += stride) {
     const RawParticle cur = raw;
-    if (p + stride < n) raw = load_particle(D, p + stride);
+    if (p + stride < nn) raw = load_particle32(D, p + stride);
     float W;
     double B;
     ParticleGrads g{};
@@ -714,16 +717,16 @@ __global__ void __launch_bounds__(kThreads) binned_adj_gather(ParticlesDev D, si
     const double gw = g.gw, gux = g.gu[0], guy = g.gu[1], guz = g.gu[2];
     const double gbx = g.gb[0], gby = g.gb[1], gbz = g.gb[2];
     const double gex = g.ge[0], gey = g.ge[1], gez = g.ge[2];
-    if (G.gw) G.gw[p] = static_cast<float>(gw);
-    if (G.gu[0]) G.gu[0][p] = static_cast<float>(gux);
-    if (G.gu[1]) G.gu[1][p] = static_cast<float>(guy);
-    if (G.gu[2]) G.gu[2][p] = static_cast<float>(guz);
-    if (G.gb[0]) G.gb[0][p] = static_cast<float>(gbx);
-    if (G.gb[1]) G.gb[1][p] = static_cast<float>(gby);
-    if (G.gb[2]) G.gb[2][p] = static_cast<float>(gbz);
-    if (G.ge[0]) G.ge[0][p] = static_cast<float>(gex);
-    if (G.ge[1]) G.ge[1][p] = static_cast<float>(gey);
-    if (G.ge[2]) G.ge[2][p] = static_cast<float>(gez);
+    if (G.gw) st32(G.gw, p, static_cast<float>(gw));
+    if (G.gu[0]) st32(G.gu[0], p, static_cast<float>(gux));
+    if (G.gu[1]) st32(G.gu[1], p, static_cast<float>(guy));
+    if (G.gu[2]) st32(G.gu[2], p, static_cast<float>(guz));
+    if (G.gb[0]) st32(G.gb[0], p, static_cast<float>(gbx));
+    if (G.gb[1]) st32(G.gb[1], p, static_cast<float>(gby));
+    if (G.gb[2]) st32(G.gb[2], p, static_cast<float>(gbz));
+    if (G.ge[0]) st32(G.ge[0], p, static_cast<float>(gex));
+    if (G.ge[1]) st32(G.ge[1], p, static_cast<float>(gey));
+    if (G.ge[2]) st32(G.ge[2], p, static_cast<float>(gez));
   }
 }
 
@@ -742,7 +745,7 @@ static size_t subhist_smem(const SyncPlan& plan) {
 }
 
 void sync_forward_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, float* out, hipStream_t st) {
-  RGN_REQUIRE(plan.binned_ok && n < (size_t{1} << 31), RGN_ERR_UNSUPPORTED, "plan outside the binned limits");
+  RGN_REQUIRE(plan.binned_ok && n < kBinnedMaxParticles, RGN_ERR_UNSUPPORTED, "plan outside the binned limits");
   const BinnedDev P = binned_view(plan);
   if (plan.NK == 0 || n == 0) {
     RGN_HIP(hipMemsetAsync(out, 0, sizeof(float) * plan.nbins, st));
@@ -810,7 +813,7 @@ void sync_forward_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, 
 
 void sync_adjoint_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, const float* gbar, float* gw,
                          float* const gu[3], float* const gb[3], float* const ge[3], hipStream_t st) {
-  RGN_REQUIRE(plan.binned_ok && n < (size_t{1} << 31), RGN_ERR_UNSUPPORTED, "plan outside the binned limits");
+  RGN_REQUIRE(plan.binned_ok && n < kBinnedMaxParticles, RGN_ERR_UNSUPPORTED, "plan outside the binned limits");
   const BinnedDev P = binned_view(plan);
   GradPtrs G{};
   G.gw = gw;

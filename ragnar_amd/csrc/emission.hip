@@ -507,8 +507,8 @@ void launch_combine(const SyncPlan& plan, const double* partials, int grid, floa
 //   binned forward  ~ 60 us + n * 13.5 ps,   binned adjoint ~ 28 us + n * 14 ps.
 // RAGNAR_SYNC_METHOD (direct | binned) overrides the automatic choice.
 int resolve_method(const SyncPlan& plan, size_t n, int method, bool adjoint) {
-  // particle indices of the factorised kernels are 32-bit
-  const bool binned_fits = plan.binned_ok && n < (size_t{1} << 31);
+  // the factorised kernels address particles with 32-bit byte offsets
+  const bool binned_fits = plan.binned_ok && n < kBinnedMaxParticles;
   if (method == RGN_METHOD_AUTO) {
     if (const char* m = std::getenv("RAGNAR_SYNC_METHOD")) {
       if (std::string(m) == "direct") return RGN_METHOD_DIRECT;
@@ -525,7 +525,7 @@ int resolve_method(const SyncPlan& plan, size_t n, int method, bool adjoint) {
   RGN_REQUIRE(method == RGN_METHOD_DIRECT || method == RGN_METHOD_BINNED, RGN_ERR_INVALID,
               "unknown method");
   RGN_REQUIRE(method != RGN_METHOD_BINNED || binned_fits, RGN_ERR_UNSUPPORTED,
-              "the binned method needs fewer than 2^31 particles, at most 1024 distinct bin "
+              "the binned method needs fewer than 2^29 particles, at most 1024 distinct bin "
               "offsets and a bin range of at most 3072 table units");
   return method;
 }

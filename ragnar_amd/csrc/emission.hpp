@@ -68,6 +68,7 @@ inline EmitConst emit_const(const SyncPlan& p) {
 // kernels.
 constexpr int kBinnedMaxM = 1024;
 constexpr int kBinnedMaxNK = 3072;
+constexpr size_t kBinnedMaxParticles = size_t{1} << 29;  // 32-bit byte offsets of 8-byte records
 
 // Per photon bin of the factorised form.
 struct BinInfo {
