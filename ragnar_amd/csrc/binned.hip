@@ -832,8 +832,9 @@ void sync_adjoint_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, 
   }
   BinnedDev Pg = P;
   if (plan.NK == 0) Pg.NK = 0;  // every particle falls outside: gradients are zero
+  const size_t bpc = static_cast<size_t>(env_int("RAGNAR_GATHER_BPC", 8));  // blocks per CU
   const int grid = static_cast<int>(std::max<size_t>(1, std::min<size_t>(ceil_div(n, kThreads),
-                                                                            static_cast<size_t>(num_cus()) * 8)));
+                                                                            static_cast<size_t>(num_cus()) * bpc)));
   const size_t smem = sizeof(double) * std::max(plan.M, 1) + sizeof(unsigned short) * plan.JQ;
   // the general form also for stores without E: the shorter one measured ~4%
   // slower here (more resident waves, same memory traffic)

@@ -87,8 +87,16 @@ struct HistParams {
   const double* edges;  // in LDS
   int nb;
   float emin, emax, scale;
+  float inv_emin;       // 1 / emin for the bin estimate (corrected against the edges)
   int log_bins, fourvel;
 };
+
+// 1 / x to about one ulp for finite x > 0 (hardware estimate, two Newton steps).
+__device__ __forceinline__ double rcp_hist(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  y = fma(y, fma(-x, y, 1.0), y);
+  return fma(y, fma(-x, y, 1.0), y);
+}
 
 // particles.cpp:228-252 for one particle: bin index and the float term.
 __device__ __forceinline__ void bin_particle(float x, float y, float z, const HistParams& H,
@@ -103,14 +111,25 @@ __device__ __forceinline__ void bin_particle(float x, float y, float z, const Hi
   } else if (energy >= H.emax) {
     idx = H.nb - 1;
   } else if (energy == energy) {
-    const float est = H.scale * log2f(static_cast<float>(energy) / H.emin);
+    // a float estimate of the index; the edge comparisons below make it exact
+    const float est = H.scale * __builtin_amdgcn_logf(static_cast<float>(energy) * H.inv_emin);
     idx = static_cast<int>(fminf(fmaxf(est, 0.0f), static_cast<float>(H.nb - 1)));
     while (idx < H.nb - 1 && energy >= H.edges[idx + 1]) ++idx;
     while (idx > 0 && energy < H.edges[idx]) --idx;
   } else {
     idx = H.nb - 1;  // NaN energy: the x86 reference's size_t conversion lands here
   }
-  const float term = H.log_bins ? static_cast<float>(1.0 / energy) : 1.0f;
+  // particles.cpp:247-252: (float)(1 / e) for log-spaced bins.  A refined
+  // reciprocal (about one double ulp) rounds to the same float except within
+  // one double ulp of a float rounding boundary (~1e-9 of the terms); the
+  // sums carry a 1e-6 tolerance, the counts do not depend on it.  The special
+  // values follow 1.0 / e: 0 -> inf, inf -> 0, NaN -> NaN.
+  float term = 1.0f;
+  if (H.log_bins) {
+    const bool normal = energy > 0.0 && energy < INFINITY;
+    term = normal ? static_cast<float>(rcp_hist(energy))
+                  : (energy == 0.0 ? INFINITY : (energy == INFINITY ? 0.0f : __builtin_nanf("")));
+  }
   atomicAdd(my_cnt + idx, 1u);
   atomicAdd(my_sum + idx, static_cast<double>(term));
 }
@@ -141,7 +160,7 @@ __global__ void __launch_bounds__(kThreads) hist_kernel(const float* __restrict_
   }
   for (int i = threadIdx.x; i <= nb; i += kThreads) ledges[i] = g_edges[i];
   __syncthreads();
-  const HistParams H{ledges, nb, emin, emax, scale, log_bins, fourvel};
+  const HistParams H{ledges, nb, emin, emax, scale, 1.0f / emin, log_bins, fourvel};
   const int copy = wave * qper + lane / (64 / qper);
   double* my_sum = lsum + copy * pitch;
   unsigned* my_cnt = lcnt + copy * pitch;
