@@ -696,11 +696,13 @@ __global__ void __launch_bounds__(kThreads) binned_adj_gather(ParticlesDev D, si
   const unsigned nn = static_cast<unsigned>(n);
   const unsigned stride = gridDim.x * kThreads;
   unsigned p = blockIdx.x * kThreads + threadIdx.x;
+  // particles and gradients stream through once (non-temporal); the table
+  // lookups keep the caches
   RawParticle raw{};
-  if (p < nn) raw = load_particle32(D, p);
+  if (p < nn) raw = load_particle32<true>(D, p);
   for (; p < nn; p += stride) {
     const RawParticle cur = raw;
-    if (p + stride < nn) raw = load_particle32(D, p + stride);
+    if (p + stride < nn) raw = load_particle32<true>(D, p + stride);
     float W;
     double B;
     ParticleGrads g{};
@@ -717,16 +719,16 @@ __global__ void __launch_bounds__(kThreads) binned_adj_gather(ParticlesDev D, si
     const double gw = g.gw, gux = g.gu[0], guy = g.gu[1], guz = g.gu[2];
     const double gbx = g.gb[0], gby = g.gb[1], gbz = g.gb[2];
     const double gex = g.ge[0], gey = g.ge[1], gez = g.ge[2];
-    if (G.gw) st32(G.gw, p, static_cast<float>(gw));
-    if (G.gu[0]) st32(G.gu[0], p, static_cast<float>(gux));
-    if (G.gu[1]) st32(G.gu[1], p, static_cast<float>(guy));
-    if (G.gu[2]) st32(G.gu[2], p, static_cast<float>(guz));
-    if (G.gb[0]) st32(G.gb[0], p, static_cast<float>(gbx));
-    if (G.gb[1]) st32(G.gb[1], p, static_cast<float>(gby));
-    if (G.gb[2]) st32(G.gb[2], p, static_cast<float>(gbz));
-    if (G.ge[0]) st32(G.ge[0], p, static_cast<float>(gex));
-    if (G.ge[1]) st32(G.ge[1], p, static_cast<float>(gey));
-    if (G.ge[2]) st32(G.ge[2], p, static_cast<float>(gez));
+    if (G.gw) st32<true>(G.gw, p, static_cast<float>(gw));
+    if (G.gu[0]) st32<true>(G.gu[0], p, static_cast<float>(gux));
+    if (G.gu[1]) st32<true>(G.gu[1], p, static_cast<float>(guy));
+    if (G.gu[2]) st32<true>(G.gu[2], p, static_cast<float>(guz));
+    if (G.gb[0]) st32<true>(G.gb[0], p, static_cast<float>(gbx));
+    if (G.gb[1]) st32<true>(G.gb[1], p, static_cast<float>(gby));
+    if (G.gb[2]) st32<true>(G.gb[2], p, static_cast<float>(gbz));
+    if (G.ge[0]) st32<true>(G.ge[0], p, static_cast<float>(gex));
+    if (G.ge[1]) st32<true>(G.ge[1], p, static_cast<float>(gey));
+    if (G.ge[2]) st32<true>(G.ge[2], p, static_cast<float>(gez));
   }
 }
 

@@ -290,13 +290,26 @@ __device__ __forceinline__ float load_or_zero(const float* p, size_t i) {
 // 2^32): the compiler then addresses with the uniform 64-bit base in scalar
 // registers plus a 32-bit per-lane offset, instead of 64-bit per-lane
 // address arithmetic for every access.
-template <class T>
+// NT = true marks a non-temporal access (scalar types only): data streamed
+// through once, which the caches then hold only briefly, leaving them to the
+// tables that other lanes re-read.
+template <bool NT = false, class T>
 __device__ __forceinline__ T ld32(const T* base, unsigned i) {
-  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * static_cast<unsigned>(sizeof(T)));
+  const T* a = reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * static_cast<unsigned>(sizeof(T)));
+  if constexpr (NT) {
+    return __builtin_nontemporal_load(a);
+  } else {
+    return *a;
+  }
 }
-template <class T>
+template <bool NT = false, class T>
 __device__ __forceinline__ void st32(T* base, unsigned i, T v) {
-  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * static_cast<unsigned>(sizeof(T))) = v;
+  T* a = reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * static_cast<unsigned>(sizeof(T)));
+  if constexpr (NT) {
+    __builtin_nontemporal_store(v, a);
+  } else {
+    *a = v;
+  }
 }
 
 __device__ __forceinline__ RawParticle load_particle(const ParticlesDev& D, size_t p) {
@@ -315,18 +328,19 @@ __device__ __forceinline__ RawParticle load_particle(const ParticlesDev& D, size
 }
 
 // load_particle for stores of fewer than 2^29 particles (32-bit byte offsets).
+template <bool NT = false>
 __device__ __forceinline__ RawParticle load_particle32(const ParticlesDev& D, unsigned p) {
   RawParticle r;
-  r.ux = ld32(D.u[0], p);
-  r.uy = ld32(D.u[1], p);
-  r.uz = ld32(D.u[2], p);
-  r.ex = D.e[0] ? ld32(D.e[0], p) : 0.0f;
-  r.ey = D.e[1] ? ld32(D.e[1], p) : 0.0f;
-  r.ez = D.e[2] ? ld32(D.e[2], p) : 0.0f;
-  r.bx = ld32(D.b[0], p);
-  r.by = ld32(D.b[1], p);
-  r.bz = ld32(D.b[2], p);
-  r.w = D.w ? ld32(D.w, p) : 1.0f;
+  r.ux = ld32<NT>(D.u[0], p);
+  r.uy = ld32<NT>(D.u[1], p);
+  r.uz = ld32<NT>(D.u[2], p);
+  r.ex = D.e[0] ? ld32<NT>(D.e[0], p) : 0.0f;
+  r.ey = D.e[1] ? ld32<NT>(D.e[1], p) : 0.0f;
+  r.ez = D.e[2] ? ld32<NT>(D.e[2], p) : 0.0f;
+  r.bx = ld32<NT>(D.b[0], p);
+  r.by = ld32<NT>(D.b[1], p);
+  r.bz = ld32<NT>(D.b[2], p);
+  r.w = D.w ? ld32<NT>(D.w, p) : 1.0f;
   return r;
 }
 

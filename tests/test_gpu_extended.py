@@ -187,15 +187,19 @@ def test_contraction_fp32_rows(orc, table):
     assert np.max(np.abs(S[0][live] / S0[live] - 1)) < 1e-5
 
 
-def test_adjoint_methods_agree_large(orc, table):
+@pytest.mark.parametrize("weighted", [False, True])
+def test_adjoint_methods_agree_large(orc, table, weighted):
+    """Both adjoint methods at a size where every thread handles several
+    particles (the loops' prefetch paths), with unit and per-particle weights."""
     n, nb = 1 << 20, 256
     u, b = workloads.particles(n, seed=12)
     bins = orc.logspace(1e-2, 1e7, nb)
     plan = T.SyncPlan(table, bins)
     U, B = dev(u), dev(b)
+    W = dev(np.random.default_rng(8).uniform(0.5, 2.0, n).astype(np.float32)) if weighted else None
     g = torch.tensor(_loss_weights(nb, seed=9), dtype=torch.float32, device=DEV)
-    d = T.sync_adjoint(plan, U, B, None, None, g, need=("w", "u", "b"), method="direct")
-    q = T.sync_adjoint(plan, U, B, None, None, g, need=("w", "u", "b"), method="binned")
+    d = T.sync_adjoint(plan, U, B, W, None, g, need=("w", "u", "b"), method="direct")
+    q = T.sync_adjoint(plan, U, B, W, None, g, need=("w", "u", "b"), method="binned")
     for name, x, y in zip("wub", d[:3], q[:3]):
         x = x.double().cpu().numpy().ravel()
         y = y.double().cpu().numpy().ravel()
