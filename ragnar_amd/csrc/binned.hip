@@ -131,6 +131,7 @@ constexpr int kSubThreads = 512;           // threads of a sub-histogram work it
 constexpr int kBatch = 4;                  // sources per lane with loads in flight together
 constexpr int kWin = 2048;                 // runs per LDS window of a work item
 constexpr int kPrefThreads = 1024;         // 4 item groups x 256 sub-cells per prefix block
+constexpr int kRowThreads = 1024;          // one row of run descriptors per scan block
 
 struct BinnedDev {
   const double* alpha;
@@ -305,20 +306,20 @@ __global__ void __launch_bounds__(kThreads) binned_bucket(ParticlesDev D, unsign
 // One exclusive scan over blocks of (run length, non-empty flag) packed in 64 bits.
 // The descriptors read are zeroed again, so the table needs no clearing pass
 // before the next call.
-__global__ void __launch_bounds__(kThreads) binned_scan_rows(unsigned* __restrict__ runs, int G,
+__global__ void __launch_bounds__(kRowThreads) binned_scan_rows(unsigned* __restrict__ runs, int G,
                                                              unsigned* __restrict__ cvoff,
                                                              unsigned* __restrict__ cidx,
                                                              unsigned* __restrict__ nrun,
                                                              unsigned* __restrict__ totals) {
-  __shared__ unsigned long long part[kThreads / 64];
+  __shared__ unsigned long long part[kRowThreads / 64];
   const int k = blockIdx.x;
   unsigned* row = runs + static_cast<size_t>(k) * G;
   unsigned* vo = cvoff + static_cast<size_t>(k) * (G + 1);
   unsigned* ci = cidx + static_cast<size_t>(k) * G;
-  const int per = (G + kThreads - 1) / kThreads;
+  const int per = (G + kRowThreads - 1) / kRowThreads;
   const int i0 = min(G, static_cast<int>(threadIdx.x) * per), i1 = min(G, i0 + per);
   // up to kRowCache descriptors per thread stay in registers between the passes
-  constexpr int kRowCache = 32;
+  constexpr int kRowCache = 8;
   unsigned cache[kRowCache];
   unsigned long long acc = 0;
   if (per <= kRowCache) {
@@ -335,7 +336,7 @@ __global__ void __launch_bounds__(kThreads) binned_scan_rows(unsigned* __restric
     }
   }
   unsigned long long all;
-  acc = block_prefix<unsigned long long, kThreads>(acc, all, part);
+  acc = block_prefix<unsigned long long, kRowThreads>(acc, all, part);
   auto emit = [&](int i, unsigned r) {
     const unsigned len = run_len(r);
     if (len) {
@@ -354,7 +355,7 @@ __global__ void __launch_bounds__(kThreads) binned_scan_rows(unsigned* __restric
   } else {
     for (int i = i0; i < i1; ++i) emit(i, row[i]);
   }
-  if (threadIdx.x == kThreads - 1) {
+  if (threadIdx.x == kRowThreads - 1) {
     const unsigned nr = static_cast<unsigned>(all >> 32);
     const unsigned total = static_cast<unsigned>(all);
     vo[nr] = total;
@@ -548,15 +549,25 @@ __global__ void __launch_bounds__(kPrefThreads) binned_prefix(int M, const unsig
   const int grp = threadIdx.x >> 8, col = threadIdx.x & 255;
   for (int j0 = 0; j0 < M1; j0 += 256) {
     const int j = j0 + col;
-    double a = 0.0, b = 0.0;
+    // this group's items it = i0 + grp + 4 q, four of them per pass with their
+    // loads in flight together, summed in a fixed order
+    double a[4] = {0.0, 0.0, 0.0, 0.0}, b[4] = {0.0, 0.0, 0.0, 0.0};
     if (j < M1) {
-      for (unsigned it = i0 + grp; it < i1; it += 4) {
-        a += partial[static_cast<size_t>(it) * 2 * M1 + j];
-        b += partial[static_cast<size_t>(it) * 2 * M1 + M1 + j];
+      unsigned it = i0 + grp;
+      for (; it + 12 < i1; it += 16) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          a[q] += partial[static_cast<size_t>(it + 4 * q) * 2 * M1 + j];
+          b[q] += partial[static_cast<size_t>(it + 4 * q) * 2 * M1 + M1 + j];
+        }
+      }
+      for (int q = 0; it < i1; it += 4, ++q) {  // at most three left
+        a[q] += partial[static_cast<size_t>(it) * 2 * M1 + j];
+        b[q] += partial[static_cast<size_t>(it) * 2 * M1 + M1 + j];
       }
     }
-    red[0][grp][col] = a;
-    red[1][grp][col] = b;
+    red[0][grp][col] = (a[0] + a[1]) + (a[2] + a[3]);
+    red[1][grp][col] = (b[0] + b[1]) + (b[2] + b[3]);
     __syncthreads();
     if (grp == 0 && j < M1) {
       s0[j] = ((red[0][0][col] + red[0][1][col]) + red[0][2][col]) + red[0][3][col];
@@ -794,7 +805,7 @@ void sync_forward_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, 
   }
   RGN_HIP(hipGetLastError());
   stage_mark(st, "binned_bucket");
-  binned_scan_rows<<<plan.NK, kThreads, 0, st>>>(runs, G, cvoff, cidx, nrun, totals);
+  binned_scan_rows<<<plan.NK, kRowThreads, 0, st>>>(runs, G, cvoff, cidx, nrun, totals);
   RGN_HIP(hipGetLastError());
   runs_clean = clean_after;
   stage_mark(st, "binned_scan_rows");
