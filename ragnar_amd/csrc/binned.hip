@@ -364,26 +364,52 @@ __global__ void __launch_bounds__(kRowThreads) binned_scan_rows(unsigned* __rest
   }
 }
 
-// Pass 2b: work items per bucket (one block): istart[k] = first item of bucket
-// k, istart[NK] = number of items.  A separate launch: folding it into the
-// last block of pass 2a needs an agent-scope release per block, which writes
-// back the XCD's L2 and measured slower.
-__global__ void __launch_bounds__(kThreads) binned_scan_items(const unsigned* __restrict__ totals, int NK,
-                                                              unsigned* __restrict__ istart) {
-  __shared__ unsigned items[kBinnedMaxNK];
-  __shared__ unsigned part[kThreads];
-  for (int k = threadIdx.x; k < NK; k += kThreads) items[k] = (totals[k] + kItem - 1) / kItem;
+// Work items: bucket k is cut into ceil(totals[k] / kItem) items of kItem
+// sources, numbered bucket after bucket.  Both helpers are block-wide (every
+// thread of the NT-thread block calls them; wsum: NT / 64 words of LDS).
+__device__ __forceinline__ unsigned items_of(unsigned total) { return (total + kItem - 1) / kItem; }
+
+// Bucket k of item `it` and the bucket's first item; false past the last item
+// (the same answer in every thread).
+template <int NT>
+__device__ __forceinline__ bool item_bucket(const unsigned* __restrict__ totals, int NK, unsigned it, int& k,
+                                            unsigned& first, unsigned* wsum, unsigned* res) {
+  const int per = (NK + NT - 1) / NT;
+  const int q0 = min(NK, static_cast<int>(threadIdx.x) * per), q1 = min(NK, q0 + per);
+  unsigned c = 0;
+  for (int q = q0; q < q1; ++q) c += items_of(totals[q]);
+  unsigned all;
+  unsigned acc = block_prefix<unsigned, NT>(c, all, wsum);
+  for (int q = q0; q < q1; ++q) {
+    const unsigned m = items_of(totals[q]);
+    if (acc <= it && it < acc + m) {
+      res[0] = static_cast<unsigned>(q);
+      res[1] = acc;
+    }
+    acc += m;
+  }
   __syncthreads();
-  const unsigned total = block_scan_exclusive<unsigned>(items, istart, NK, part);
-  if (threadIdx.x == 0) istart[NK] = total;
+  if (it >= all) return false;
+  k = static_cast<int>(res[0]);
+  first = res[1];
+  return true;
+}
+
+// First item of bucket k (the number of items of buckets before it).
+template <int NT>
+__device__ __forceinline__ unsigned items_before(const unsigned* __restrict__ totals, int k, unsigned* wsum) {
+  unsigned c = 0;
+  for (int q = threadIdx.x; q < k; q += NT) c += items_of(totals[q]);
+  unsigned all;
+  block_prefix<unsigned, NT>(c, all, wsum);
+  return all;
 }
 
 // Pass 3: sub-histogram of one work item: sources [v0, v1) of bucket k, found
 // as runs of the bucket blocks' sorted chunks.  Each wave takes the non-empty
 // runs of 64 consecutive blocks at a time (ballot) and reads every run
 // coalesced, four loads in flight per lane.
-__global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G, const unsigned* __restrict__ istart,
-                                                              const unsigned* __restrict__ totals,
+__global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G, const unsigned* __restrict__ totals,
                                                               const unsigned* __restrict__ cvoff,
                                                               const unsigned* __restrict__ cidx,
                                                               const unsigned* __restrict__ nrun,
@@ -398,10 +424,12 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
   unsigned* wro = reinterpret_cast<unsigned*>(athr + P.M);   // kWin + 1
   unsigned* wlo = wro + kWin + 1;                              // kWin
   unsigned short* jlut = reinterpret_cast<unsigned short*>(wlo + kWin);
-  __shared__ unsigned s_cnt[3];
+  __shared__ unsigned s_cnt[2], s_item[2], s_wsum[kWaves];
   const unsigned it = blockIdx.x;
-  if (it >= istart[P.NK]) return;  // uniform per block
-  if (threadIdx.x < 3) s_cnt[threadIdx.x] = 0u;
+  int k;
+  unsigned first;
+  if (!item_bucket<kSubThreads>(totals, P.NK, it, k, first, s_wsum, s_item)) return;  // uniform per block
+  if (threadIdx.x < 2) s_cnt[threadIdx.x] = 0u;
   for (int i = threadIdx.x; i < M1; i += kSubThreads) {
     m0[i] = 0.0;
     m1[i] = 0.0;
@@ -409,21 +437,13 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
   for (int i = threadIdx.x; i < P.M; i += kSubThreads) athr[i] = P.athr[i];
   for (int i = threadIdx.x; i < P.JQ; i += kSubThreads) jlut[i] = P.jlut[i];
   __syncthreads();
-  // Monotone searches as counts (all loads of a search in flight at once):
-  // bucket of this item k = #{k' : istart[k'] <= it} - 1.
-  {
-    unsigned c = 0;
-    for (int q = threadIdx.x; q < P.NK; q += kSubThreads) c += istart[q] <= it;
-    if (c) atomicAdd(&s_cnt[0], c);
-  }
-  __syncthreads();
-  const int k = static_cast<int>(s_cnt[0]) - 1;
-  const unsigned v0 = (it - istart[k]) * kItem;
+  const unsigned v0 = (it - first) * kItem;
   const unsigned v1 = min(totals[k], v0 + kItem);
   const unsigned* vo = cvoff + static_cast<size_t>(k) * (G + 1);
   const unsigned* ci = cidx + static_cast<size_t>(k) * G;
   const int nr = static_cast<int>(nrun[k]);
-  // first and last run of the item: #{i < nr : vo[i] <= v} - 1 (vo strictly
+  // first and last run of the item, a monotone search done as a count (all
+  // loads in flight at once): #{i < nr : vo[i] <= v} - 1 (vo strictly
   // increases: every listed run is non-empty)
   {
     unsigned c0 = 0, c1 = 0;
@@ -432,11 +452,11 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
       c0 += r <= v0;
       c1 += r <= v1 - 1;
     }
-    if (c0) atomicAdd(&s_cnt[1], c0);
-    if (c1) atomicAdd(&s_cnt[2], c1);
+    if (c0) atomicAdd(&s_cnt[0], c0);
+    if (c1) atomicAdd(&s_cnt[1], c1);
   }
   __syncthreads();
-  const int rlo = static_cast<int>(s_cnt[1]) - 1, rhi = static_cast<int>(s_cnt[2]) - 1;
+  const int rlo = static_cast<int>(s_cnt[0]) - 1, rhi = static_cast<int>(s_cnt[1]) - 1;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int ws = rlo; ws <= rhi; ws += kWin) {
     const int nc = min(rhi - ws + 1, kWin);
@@ -535,7 +555,7 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
 // Pass 4: per bucket, its items summed (four interleaved item groups, combined
 // in a fixed order), then inclusive prefix sums over sub-cells:
 // P0[k][j] = sum_{j' <= j} m0, P1 likewise.
-__global__ void __launch_bounds__(kPrefThreads) binned_prefix(int M, const unsigned* __restrict__ istart,
+__global__ void __launch_bounds__(kPrefThreads) binned_prefix(int M, const unsigned* __restrict__ totals,
                                                               const double* __restrict__ partial,
                                                               double* __restrict__ P0, double* __restrict__ P1) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -545,7 +565,8 @@ __global__ void __launch_bounds__(kPrefThreads) binned_prefix(int M, const unsig
   double* s0 = reinterpret_cast<double*>(smem);
   double* s1 = s0 + M1;
   const int k = blockIdx.x;
-  const unsigned i0 = istart[k], i1 = istart[k + 1];
+  __shared__ unsigned s_wsum[kPrefThreads / 64];
+  const unsigned i0 = items_before<kPrefThreads>(totals, k, s_wsum), i1 = i0 + items_of(totals[k]);
   const int grp = threadIdx.x >> 8, col = threadIdx.x & 255;
   for (int j0 = 0; j0 < M1; j0 += 256) {
     const int j = j0 + col;
@@ -693,7 +714,7 @@ struct GradPtrs {
   float* ge[3];
 };
 
-template <bool HAS_E>
+template <bool HAS_E, bool NT>
 __global__ void __launch_bounds__(kThreads) binned_adj_gather(ParticlesDev D, size_t n, BinnedDev P,
                                                               const double2* __restrict__ U, GradPtrs G) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -707,13 +728,13 @@ __global__ void __launch_bounds__(kThreads) binned_adj_gather(ParticlesDev D, si
   const unsigned nn = static_cast<unsigned>(n);
   const unsigned stride = gridDim.x * kThreads;
   unsigned p = blockIdx.x * kThreads + threadIdx.x;
-  // particles and gradients stream through once (non-temporal); the table
-  // lookups keep the caches
+  // NT: particles and gradients stream through once (non-temporal), so that
+  // the table lookups keep the caches
   RawParticle raw{};
-  if (p < nn) raw = load_particle32<true>(D, p);
+  if (p < nn) raw = load_particle32<NT>(D, p);
   for (; p < nn; p += stride) {
     const RawParticle cur = raw;
-    if (p + stride < nn) raw = load_particle32<true>(D, p + stride);
+    if (p + stride < nn) raw = load_particle32<NT>(D, p + stride);
     float W;
     double B;
     ParticleGrads g{};
@@ -730,16 +751,16 @@ __global__ void __launch_bounds__(kThreads) binned_adj_gather(ParticlesDev D, si
     const double gw = g.gw, gux = g.gu[0], guy = g.gu[1], guz = g.gu[2];
     const double gbx = g.gb[0], gby = g.gb[1], gbz = g.gb[2];
     const double gex = g.ge[0], gey = g.ge[1], gez = g.ge[2];
-    if (G.gw) st32<true>(G.gw, p, static_cast<float>(gw));
-    if (G.gu[0]) st32<true>(G.gu[0], p, static_cast<float>(gux));
-    if (G.gu[1]) st32<true>(G.gu[1], p, static_cast<float>(guy));
-    if (G.gu[2]) st32<true>(G.gu[2], p, static_cast<float>(guz));
-    if (G.gb[0]) st32<true>(G.gb[0], p, static_cast<float>(gbx));
-    if (G.gb[1]) st32<true>(G.gb[1], p, static_cast<float>(gby));
-    if (G.gb[2]) st32<true>(G.gb[2], p, static_cast<float>(gbz));
-    if (G.ge[0]) st32<true>(G.ge[0], p, static_cast<float>(gex));
-    if (G.ge[1]) st32<true>(G.ge[1], p, static_cast<float>(gey));
-    if (G.ge[2]) st32<true>(G.ge[2], p, static_cast<float>(gez));
+    if (G.gw) st32<NT>(G.gw, p, static_cast<float>(gw));
+    if (G.gu[0]) st32<NT>(G.gu[0], p, static_cast<float>(gux));
+    if (G.gu[1]) st32<NT>(G.gu[1], p, static_cast<float>(guy));
+    if (G.gu[2]) st32<NT>(G.gu[2], p, static_cast<float>(guz));
+    if (G.gb[0]) st32<NT>(G.gb[0], p, static_cast<float>(gbx));
+    if (G.gb[1]) st32<NT>(G.gb[1], p, static_cast<float>(gby));
+    if (G.gb[2]) st32<NT>(G.gb[2], p, static_cast<float>(gbz));
+    if (G.ge[0]) st32<NT>(G.ge[0], p, static_cast<float>(gex));
+    if (G.ge[1]) st32<NT>(G.ge[1], p, static_cast<float>(gey));
+    if (G.ge[2]) st32<NT>(G.ge[2], p, static_cast<float>(gez));
   }
 }
 
@@ -774,10 +795,9 @@ void sync_forward_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, 
   unsigned* runs = static_cast<unsigned*>(scratch(sizeof(unsigned) * NK * G, 8, &fresh));
   unsigned* cvoff = static_cast<unsigned*>(scratch(sizeof(unsigned) * NK * (G + 1), 10));
   unsigned* cidx = static_cast<unsigned*>(scratch(sizeof(unsigned) * NK * G, 9));
-  unsigned* small = static_cast<unsigned*>(scratch(sizeof(unsigned) * (3 * (NK + 1) + 64), 11));
+  unsigned* small = static_cast<unsigned*>(scratch(sizeof(unsigned) * (2 * (NK + 1) + 64), 11));
   unsigned* totals = small;
-  unsigned* istart = small + (NK + 1);
-  unsigned* nrun = small + 2 * (NK + 1);
+  unsigned* nrun = small + (NK + 1);
   double* partial = static_cast<double*>(scratch(sizeof(double) * 2 * M1 * max_items, 12));
   double* pref = static_cast<double*>(scratch(sizeof(double) * 2 * M1 * NK, 13));
   double* P0 = pref;
@@ -809,14 +829,11 @@ void sync_forward_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, 
   RGN_HIP(hipGetLastError());
   runs_clean = clean_after;
   stage_mark(st, "binned_scan_rows");
-  binned_scan_items<<<1, kThreads, 0, st>>>(totals, plan.NK, istart);
-  RGN_HIP(hipGetLastError());
-  stage_mark(st, "binned_scan_items");
   binned_subhist<<<static_cast<unsigned>(max_items), kSubThreads, subhist_smem(plan), st>>>(
-    P, G, istart, totals, cvoff, cidx, nrun, sortWP, partial);
+    P, G, totals, cvoff, cidx, nrun, sortWP, partial);
   RGN_HIP(hipGetLastError());
   stage_mark(st, "binned_subhist");
-  binned_prefix<<<plan.NK, kPrefThreads, sizeof(double) * 2 * M1, st>>>(plan.M, istart, partial, P0, P1);
+  binned_prefix<<<plan.NK, kPrefThreads, sizeof(double) * 2 * M1, st>>>(plan.M, totals, partial, P0, P1);
   RGN_HIP(hipGetLastError());
   stage_mark(st, "binned_prefix");
   binned_evaluate<<<plan.nbins, kThreads, 0, st>>>(P, P0, P1, out);
@@ -849,9 +866,18 @@ void sync_adjoint_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, 
   const int grid = static_cast<int>(std::max<size_t>(1, std::min<size_t>(ceil_div(n, kThreads),
                                                                             static_cast<size_t>(num_cus()) * bpc)));
   const size_t smem = sizeof(double) * std::max(plan.M, 1) + sizeof(unsigned short) * plan.JQ;
-  // the general form also for stores without E: the shorter one measured ~4%
-  // slower here (more resident waves, same memory traffic)
-  binned_adj_gather<true><<<grid, kThreads, smem, st>>>(D, n, Pg, U, G);
+  // The general form also for stores without E: the shorter one measured ~4%
+  // slower here (more resident waves, same memory traffic).  Non-temporal
+  // streams pay off once a call's particles and gradients outgrow the
+  // device's last-level cache (256 MiB on MI355X): below that, inputs written
+  // just before and outputs read just after are served from it.
+  const size_t streams = 6 + (D.e[0] ? 3 : 0) + (D.w ? 1 : 0) + (G.gw ? 1 : 0) + (G.gu[0] ? 3 : 0) +
+                         (G.gb[0] ? 3 : 0) + (G.ge[0] ? 3 : 0);
+  if (sizeof(float) * streams * n > (size_t{256} << 20)) {
+    binned_adj_gather<true, true><<<grid, kThreads, smem, st>>>(D, n, Pg, U, G);
+  } else {
+    binned_adj_gather<true, false><<<grid, kThreads, smem, st>>>(D, n, Pg, U, G);
+  }
   RGN_HIP(hipGetLastError());
   stage_mark(st, "binned_adj_gather");
 }
