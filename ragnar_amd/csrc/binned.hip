@@ -16,13 +16,13 @@
 //
 // Forward (a scatter), no global atomics (device-scope FP64 atomics run at
 // ~14 G/s on MI355X):
-//   bucket    each block of 4096 particles computes (k, W, phi), counting-sorts
+//   bucket    each block of 8192 particles computes (k, W, phi), counting-sorts
 //             its sources by k in LDS and writes the sorted (W, phi) chunk
 //             coalesced, plus one run descriptor per non-empty bucket;
 //   scan      per bucket, its non-empty runs compacted in block order with
-//             their offsets within the bucket; then work items of up to kItem
-//             sources per bucket;
-//   subhist   each item walks its runs and reduces the sources into M + 1
+//             their offsets within the bucket;
+//   subhist   each bucket is cut into work items of up to kItem sources; each
+//             item walks its runs and reduces the sources into M + 1
 //             sub-cells in LDS (ds_add_f64);
 //   prefix    per bucket: the items in order, then prefix sums over sub-cells;
 //   evaluate  one block per photon bin, fixed-order reduction over k.
@@ -124,8 +124,10 @@ void build_binned_tables(SyncPlan& plan, const std::vector<float>& xs, const std
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int kBucketThreads = 512;        // threads of a bucket block (two per CU)
 constexpr int kPer = 16;                   // particles per thread in the bucket pass
-constexpr int kChunk = kThreads * kPer;    // particles per bucket block
+constexpr int kChunk = kBucketThreads * kPer;  // particles per bucket block
+static_assert(kChunk <= 32768, "run descriptors pack start and length in 16 bits each");
 constexpr unsigned kItem = 16384;          // sources per sub-histogram work item
 constexpr int kSubThreads = 512;           // threads of a sub-histogram work item
 constexpr int kBatch = 4;                  // sources per lane with loads in flight together
@@ -227,7 +229,7 @@ __device__ T block_scan_exclusive(PA a, PO o, int N, T* part) {
 }
 
 // Run descriptor of bucket k in bucket block b: start << 16 | length (both at
-// most kChunk = 4096).  Only non-empty runs are written (each scattered 4-byte
+// most kChunk = 8192).  Only non-empty runs are written (each scattered 4-byte
 // store costs a 32-byte write-back); the scan pass zeroes what it reads, so the
 // table stays clear between calls and fill_zero runs only on a new buffer.
 __device__ __forceinline__ unsigned run_len(unsigned r) { return r & 0xffffu; }
@@ -244,16 +246,16 @@ __global__ void __launch_bounds__(kThreads) fill_zero(unsigned* __restrict__ p, 
 //   sortWP[b * kChunk + i]: the block's sources (W, phi) in bucket order;
 //   runs[k * G + b]: run descriptor of bucket k in block b.
 template <bool HAS_E>
-__global__ void __launch_bounds__(kThreads) binned_bucket(ParticlesDev D, unsigned n, BinnedDev P,
+__global__ void __launch_bounds__(kBucketThreads) binned_bucket(ParticlesDev D, unsigned n, BinnedDev P,
                                                           float2* __restrict__ sortWP, unsigned* __restrict__ runs) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int NK = P.NK;
   unsigned* hist = reinterpret_cast<unsigned*>(smem);
   unsigned* off = hist + NK;
   unsigned* part = off + NK;
-  float* sW = reinterpret_cast<float*>(part + kThreads);
+  float* sW = reinterpret_cast<float*>(part + kBucketThreads);
   float* sP = sW + kChunk;
-  for (int i = threadIdx.x; i < NK; i += kThreads) hist[i] = 0u;
+  for (int i = threadIdx.x; i < NK; i += kBucketThreads) hist[i] = 0u;
   __syncthreads();
   const unsigned G = gridDim.x;
   const unsigned base = blockIdx.x * static_cast<unsigned>(kChunk);
@@ -264,9 +266,9 @@ __global__ void __launch_bounds__(kThreads) binned_bucket(ParticlesDev D, unsign
   RawParticle raw{};
   if (p < n) raw = load_particle32(D, p);
 #pragma unroll
-  for (int i = 0; i < kPer; ++i, p += kThreads) {
+  for (int i = 0; i < kPer; ++i, p += kBucketThreads) {
     const RawParticle cur = raw;
-    if (i + 1 < kPer && p + kThreads < n) raw = load_particle32(D, p + kThreads);
+    if (i + 1 < kPer && p + kBucketThreads < n) raw = load_particle32(D, p + kBucketThreads);
     kk[i] = -1;
     Wr[i] = 0.0f;
     ph[i] = 0.0f;
@@ -282,8 +284,8 @@ __global__ void __launch_bounds__(kThreads) binned_bucket(ParticlesDev D, unsign
     }
   }
   __syncthreads();
-  block_scan_exclusive<unsigned>(hist, off, NK, part);
-  for (int k = threadIdx.x; k < NK; k += kThreads) {
+  block_scan_exclusive<unsigned, kBucketThreads>(hist, off, NK, part);
+  for (int k = threadIdx.x; k < NK; k += kBucketThreads) {
     if (hist[k]) runs[static_cast<size_t>(k) * G + blockIdx.x] = off[k] << 16 | hist[k];
   }
 #pragma unroll
@@ -296,7 +298,7 @@ __global__ void __launch_bounds__(kThreads) binned_bucket(ParticlesDev D, unsign
   }
   __syncthreads();
   const unsigned total = off[NK - 1] + hist[NK - 1];
-  for (unsigned i = threadIdx.x; i < total; i += kThreads) st32(sortWP, base + i, make_float2(sW[i], sP[i]));
+  for (unsigned i = threadIdx.x; i < total; i += kBucketThreads) st32(sortWP, base + i, make_float2(sW[i], sP[i]));
 }
 
 // Pass 2a: per bucket k, its non-empty runs in block order, compacted:
@@ -490,15 +492,21 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
       // cursor, issue every load, then do the sub-cell lookups and atomics.
       // Full batches run without per-position guards.
       auto advance = [&](unsigned v) {
-        cur += crs[cur + 1] <= v;  // the usual case: at most one run boundary
-        if (crs[cur + 1] <= v) {   // several short runs crossed
-          int lo = cur + 1, hi = nc - 1;
-          while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (crs[mid] <= v) lo = mid; else hi = mid - 1;
-          }
-          cur = lo;
+        // the last run c >= cur with crs[c] <= v (crs[nc] > v): gallop from the
+        // cursor, then bisect the bracket, so that crossing d boundaries costs
+        // about 2 log2(d) reads
+        if (crs[cur + 1] > v) return;
+        int lo = cur + 1, step = 1;
+        while (lo + step < nc && crs[lo + step] <= v) {
+          lo += step;
+          step <<= 1;
         }
+        int hi = min(lo + step, nc) - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (crs[mid] <= v) lo = mid; else hi = mid - 1;
+        }
+        cur = lo;
       };
       auto accumulate = [&](float2 wp) {
         int j = jlut[min(static_cast<int>(wp.y * P.JQ), P.JQ - 1)];
@@ -769,7 +777,7 @@ __global__ void __launch_bounds__(kThreads) binned_adj_gather(ParticlesDev D, si
 // LDS bytes of the bucket pass for a plan (hist, offsets, scan partials and
 // the block's sorted sources).
 static size_t bucket_smem(const SyncPlan& plan) {
-  return sizeof(unsigned) * (2 * static_cast<size_t>(plan.NK) + kThreads) + 2 * sizeof(float) * kChunk;
+  return sizeof(unsigned) * (2 * static_cast<size_t>(plan.NK) + kBucketThreads) + 2 * sizeof(float) * kChunk;
 }
 
 static size_t subhist_smem(const SyncPlan& plan) {
@@ -819,9 +827,9 @@ void sync_forward_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, 
   runs_clean = 0;
   const bool has_e = D.e[0] || D.e[1] || D.e[2];
   if (has_e) {
-    binned_bucket<true><<<G, kThreads, bucket_smem(plan), st>>>(D, static_cast<unsigned>(n), P, sortWP, runs);
+    binned_bucket<true><<<G, kBucketThreads, bucket_smem(plan), st>>>(D, static_cast<unsigned>(n), P, sortWP, runs);
   } else {
-    binned_bucket<false><<<G, kThreads, bucket_smem(plan), st>>>(D, static_cast<unsigned>(n), P, sortWP, runs);
+    binned_bucket<false><<<G, kBucketThreads, bucket_smem(plan), st>>>(D, static_cast<unsigned>(n), P, sortWP, runs);
   }
   RGN_HIP(hipGetLastError());
   stage_mark(st, "binned_bucket");
