@@ -132,6 +132,7 @@ constexpr unsigned kItem = 16384;          // sources per sub-histogram work ite
 constexpr int kSubThreads = 512;           // threads of a sub-histogram work item
 constexpr int kBatch = 4;                  // sources per lane with loads in flight together
 constexpr int kWin = 2048;                 // runs per LDS window of a work item
+constexpr int kLutCells = 2048;            // sub-cell lookup cells in a sub-histogram block
 constexpr int kPrefThreads = 1024;         // 4 item groups x 256 sub-cells per prefix block
 constexpr int kRowThreads = 1024;          // one row of run descriptors per scan block
 
@@ -422,10 +423,11 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
   const int M1 = P.M + 1;
   double* m0 = reinterpret_cast<double*>(smem);
   double* m1 = m0 + M1;
-  float* athr = reinterpret_cast<float*>(m1 + M1);
+  const int JQ2 = min(P.JQ, kLutCells);
+  uint2* lut = reinterpret_cast<uint2*>(m1 + M1);            // JQ2
+  float* athr = reinterpret_cast<float*>(lut + JQ2);
   unsigned* wro = reinterpret_cast<unsigned*>(athr + P.M);   // kWin + 1
   unsigned* wlo = wro + kWin + 1;                              // kWin
-  unsigned short* jlut = reinterpret_cast<unsigned short*>(wlo + kWin);
   __shared__ unsigned s_cnt[2], s_item[2], s_wsum[kWaves];
   const unsigned it = blockIdx.x;
   int k;
@@ -437,7 +439,15 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
     m1[i] = 0.0;
   }
   for (int i = threadIdx.x; i < P.M; i += kSubThreads) athr[i] = P.athr[i];
-  for (int i = threadIdx.x; i < P.JQ; i += kSubThreads) jlut[i] = P.jlut[i];
+  // lookup cell c of [c / JQ2, (c + 1) / JQ2): the alphas below it, the first
+  // threshold, and whether the cell holds two alphas or more
+  for (int c = threadIdx.x; c < JQ2; c += kSubThreads) {
+    const int r = P.JQ / JQ2;
+    const unsigned j0 = P.jlut[c * r];
+    const unsigned jn = c + 1 < JQ2 ? P.jlut[(c + 1) * r] : static_cast<unsigned>(P.M);
+    const float thr = j0 < static_cast<unsigned>(P.M) ? P.athr[j0] : INFINITY;
+    lut[c] = make_uint2(j0 | (jn - j0 >= 2 ? 0x10000u : 0u), __float_as_uint(thr));
+  }
   __syncthreads();
   const unsigned v0 = (it - first) * kItem;
   const unsigned v1 = min(totals[k], v0 + kItem);
@@ -509,9 +519,11 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
         cur = lo;
       };
       auto accumulate = [&](float2 wp) {
-        int j = jlut[min(static_cast<int>(wp.y * P.JQ), P.JQ - 1)];
-        j += j < P.M && athr[j] <= wp.y;
-        while (j < P.M && athr[j] <= wp.y) ++j;  // rare: a grid cell with several alphas
+        const uint2 e = lut[min(static_cast<int>(wp.y * JQ2), JQ2 - 1)];
+        int j = static_cast<int>(e.x & 0xffffu) + (__uint_as_float(e.y) <= wp.y);
+        if (e.x >> 16) {  // rare: a cell with several alphas
+          while (j < P.M && athr[j] <= wp.y) ++j;
+        }
         const double wd = wp.x;
         atomicAdd(m0 + j, wd);
         atomicAdd(m1 + j, wd * static_cast<double>(wp.y));
@@ -782,8 +794,8 @@ static size_t bucket_smem(const SyncPlan& plan) {
 
 static size_t subhist_smem(const SyncPlan& plan) {
   const size_t M1 = plan.M + 1;
-  return sizeof(double) * 2 * M1 + sizeof(float) * plan.M + sizeof(unsigned) * (2 * kWin + 1) +
-         sizeof(unsigned short) * plan.JQ;
+  return sizeof(double) * 2 * M1 + sizeof(uint2) * std::min(plan.JQ, kLutCells) + sizeof(float) * plan.M +
+         sizeof(unsigned) * (2 * kWin + 1);
 }
 
 void sync_forward_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, float* out, hipStream_t st) {
