@@ -254,8 +254,7 @@ __global__ void __launch_bounds__(kBucketThreads) binned_bucket(ParticlesDev D, 
   unsigned* hist = reinterpret_cast<unsigned*>(smem);
   unsigned* off = hist + NK;
   unsigned* part = off + NK;
-  float* sW = reinterpret_cast<float*>(part + kBucketThreads);
-  float* sP = sW + kChunk;
+  float2* sWP = reinterpret_cast<float2*>(part + kBucketThreads);  // after 2 NK + kBucketThreads words: 8-byte aligned
   for (int i = threadIdx.x; i < NK; i += kBucketThreads) hist[i] = 0u;
   __syncthreads();
   const unsigned G = gridDim.x;
@@ -292,14 +291,12 @@ __global__ void __launch_bounds__(kBucketThreads) binned_bucket(ParticlesDev D, 
 #pragma unroll
   for (int i = 0; i < kPer; ++i) {
     if (kk[i] >= 0) {
-      const unsigned pos = off[kk[i]] + rk[i];
-      sW[pos] = Wr[i];
-      sP[pos] = ph[i];
+      sWP[off[kk[i]] + rk[i]] = make_float2(Wr[i], ph[i]);
     }
   }
   __syncthreads();
   const unsigned total = off[NK - 1] + hist[NK - 1];
-  for (unsigned i = threadIdx.x; i < total; i += kBucketThreads) st32(sortWP, base + i, make_float2(sW[i], sP[i]));
+  for (unsigned i = threadIdx.x; i < total; i += kBucketThreads) st32(sortWP, base + i, sWP[i]);
 }
 
 // Pass 2a: per bucket k, its non-empty runs in block order, compacted:
