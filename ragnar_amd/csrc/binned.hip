@@ -275,7 +275,11 @@ __global__ void __launch_bounds__(kBucketThreads) binned_bucket(ParticlesDev D, 
     rk[i] = 0u;
     float W;
     double B;
-    if (p < n && particle_wb(omega_terms<HAS_E>(cur, P.em), cur, P, W, B)) {
+    // OmegaSync with refined reciprocals (about one double ulp): chiR and
+    // e_peak round to the reference's floats except within ~2 double ulps of
+    // a float rounding boundary, where one float ulp moves W and B; the
+    // binned sum never depended on them bit for bit
+    if (p < n && particle_wb(omega_terms_approx<HAS_E>(cur, P.em), cur, P, W, B)) {
       const double fl = floor(B);
       kk[i] = static_cast<int>(fl) - P.kmin;
       Wr[i] = W;
