@@ -310,7 +310,7 @@ def roofline(prof, n, nb, plan):
       the interpolant and the accumulating FMA); bound: FP32 VALU.
     binned_bucket: reads u and B (24 B/particle) and writes each contributing
       source's (W, phi) (8 B/particle); its run descriptors (4 B per non-empty
-      run, ~36 sources per run here) are left out; bound: HBM.
+      run, ~40 sources per run here) are left out; bound: HBM.
     binned_subhist: reads the sorted (W, phi) (8 B/particle) and writes its
       per-item moments (2 x (M + 1) x 8 B per item); bound: HBM.
     """
@@ -322,7 +322,6 @@ def roofline(prof, n, nb, plan):
         return {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
                 "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": read_traffic(name),
                 "kernel": name, "kernel_ms": ms, "flop_per_eval": FLOP_PER_EVAL_FWD}
-    blocks = -(-n // 4096)
     if name == "binned_bucket":
         nbytes = 32 * n
     elif name == "binned_subhist":
