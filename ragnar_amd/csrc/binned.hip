@@ -490,34 +490,33 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
       const unsigned va = crs[0], vb = crs[nc];
       const unsigned per = ((vb - va + kWaves - 1) / kWaves + 63) & ~63u;
       const unsigned pa = min(vb, va + per * wave), pb = min(vb, pa + per);
-      int cur = 0;
-      if (pa + lane < pb) {
-        int lo = 0, hi = nc - 1;  // last run starting at or before the lane's first position
+      // The wave's positions go 64 at a time, lane l taking v00 + l.  c0 is
+      // the run holding v00 (wave-uniform).  Each lane reads one run start
+      // past c0; those inside the next 64 positions become bits of a
+      // position bitmap, and a lane's run is c0 plus the starts at or below
+      // its position: no per-lane search.
+      int c0 = 0;
+      if (pa < pb) {
+        int lo = 0, hi = nc - 1;  // last run starting at or before pa
         while (lo < hi) {
           const int mid = (lo + hi + 1) >> 1;
-          if (crs[mid] <= pa + lane) lo = mid; else hi = mid - 1;
+          if (crs[mid] <= pa) lo = mid; else hi = mid - 1;
         }
-        cur = lo;
+        c0 = lo;
       }
-      // kBatch positions per lane at a time: resolve them through the run
-      // cursor, issue every load, then do the sub-cell lookups and atomics.
-      // Full batches run without per-position guards.
-      auto advance = [&](unsigned v) {
-        // the last run c >= cur with crs[c] <= v (crs[nc] > v): gallop from the
-        // cursor, then bisect the bracket, so that crossing d boundaries costs
-        // about 2 log2(d) reads
-        if (crs[cur + 1] > v) return;
-        int lo = cur + 1, step = 1;
-        while (lo + step < nc && crs[lo + step] <= v) {
-          lo += step;
-          step <<= 1;
+      auto step = [&](unsigned v00) {  // every lane calls it; returns the lane's run
+        const int c = c0 + 1 + lane;
+        const unsigned sc = c <= nc ? crs[c] : 0xffffffffu;
+        const unsigned long long in = __ballot(sc <= v00 + 64);
+        const int L = __popcll(in);  // run starts in (v00, v00 + 64]: held by lanes 0 .. L - 1
+        unsigned long long bits = 0;
+        for (int i = 0; i < L; ++i) {
+          const unsigned si = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(sc), i));
+          if (si - v00 < 64u) bits |= 1ull << (si - v00);
         }
-        int hi = min(lo + step, nc) - 1;
-        while (lo < hi) {
-          const int mid = (lo + hi + 1) >> 1;
-          if (crs[mid] <= v) lo = mid; else hi = mid - 1;
-        }
-        cur = lo;
+        const int run = c0 + __popcll(bits & ((2ull << lane) - 1));
+        c0 += L;
+        return min(run, nc - 1);  // past the window end only for positions >= pb
       };
       auto accumulate = [&](float2 wp) {
         const uint2 e = lut[min(static_cast<int>(wp.y * JQ2), JQ2 - 1)];
@@ -535,8 +534,8 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
 #pragma unroll
         for (int q = 0; q < kBatch; ++q) {
           const unsigned v = v00 + q * 64 + lane;
-          advance(v);
-          idx[q] = cix[cur] + (v - crs[cur]);
+          const int r = step(v00 + q * 64);
+          idx[q] = cix[r] + (v - crs[r]);
         }
         float2 wp[kBatch];
 #pragma unroll
@@ -549,11 +548,8 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
 #pragma unroll
         for (int q = 0; q < kBatch; ++q) {
           const unsigned v = v00 + q * 64 + lane;
-          idx[q] = ~0u;
-          if (v < pb) {
-            advance(v);
-            idx[q] = cix[cur] + (v - crs[cur]);
-          }
+          const int r = step(v00 + q * 64);
+          idx[q] = v < pb ? cix[r] + (v - crs[r]) : ~0u;
         }
         float2 wp[kBatch];
 #pragma unroll
