@@ -410,9 +410,10 @@ __device__ __forceinline__ unsigned items_before(const unsigned* __restrict__ to
 }
 
 // Pass 3: sub-histogram of one work item: sources [v0, v1) of bucket k, found
-// as runs of the bucket blocks' sorted chunks.  Each wave takes the non-empty
-// runs of 64 consecutive blocks at a time (ballot) and reads every run
-// coalesced, four loads in flight per lane.
+// as runs of the bucket blocks' sorted chunks.  The item's runs are copied to
+// LDS; each wave walks a contiguous share of the positions, 64 per step (one
+// per lane), four loads in flight per lane, and reduces the sources into the
+// M + 1 sub-cells with LDS atomics.
 __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G, const unsigned* __restrict__ totals,
                                                               const unsigned* __restrict__ cvoff,
                                                               const unsigned* __restrict__ cidx,

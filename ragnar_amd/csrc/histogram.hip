@@ -111,11 +111,16 @@ __device__ __forceinline__ void bin_particle(float x, float y, float z, const Hi
   } else if (energy >= H.emax) {
     idx = H.nb - 1;
   } else if (energy == energy) {
-    // a float estimate of the index; the edge comparisons below make it exact
+    // a float estimate of the index; the edges make it exact.  Both edges of
+    // the estimated bin come in one LDS read; only an estimate off by a bin
+    // (energies within float rounding of an edge) walks further.
     const float est = H.scale * __builtin_amdgcn_logf(static_cast<float>(energy) * H.inv_emin);
     idx = static_cast<int>(fminf(fmaxf(est, 0.0f), static_cast<float>(H.nb - 1)));
-    while (idx < H.nb - 1 && energy >= H.edges[idx + 1]) ++idx;
-    while (idx > 0 && energy < H.edges[idx]) --idx;
+    const double lo = H.edges[idx], hi = H.edges[idx + 1];
+    if (!(lo <= energy && energy < hi)) {
+      while (idx < H.nb - 1 && energy >= H.edges[idx + 1]) ++idx;
+      while (idx > 0 && energy < H.edges[idx]) --idx;
+    }
   } else {
     idx = H.nb - 1;  // NaN energy: the x86 reference's size_t conversion lands here
   }
