@@ -54,6 +54,12 @@ def parse():
     ap.add_argument("--profile", action="store_true",
                     help="dispatch each full-size kernel once (forward, adjoint, histogram) and a "
                          "1 GiB device copy that calibrates rocprofv3 byte counters, then exit")
+    ap.add_argument("--profile-compute", action="store_true",
+                    help="dispatch the compute-bound kernels (pair-by-pair forward, split-FP16 "
+                         "contraction) for rocprofv3 VALU / MFMA counters, then exit")
+    ap.add_argument("--summarize-compute", nargs=2, metavar=("TRACE_DIR", "PMC_DIR"),
+                    help="summarise a --kernel-trace --stats run and a --pmc run of "
+                         "--profile-compute into profiles/compute_units.md, then exit")
     return ap.parse_args()
 
 
@@ -95,6 +101,142 @@ def profile_run(args):
     dst.copy_(src)
     torch.cuda.synchronize()
     print("profile: done", flush=True)
+
+
+# Counters of the --profile-compute pass: FP32 / FP64 VALU flops (MFMA excluded),
+# F16 MFMA operations / 512, and rocprofv3's derived VALUBusy and MfmaUtil
+# (5 SQ counters and GRBM_GUI_ACTIVE: one pass).
+COMPUTE_COUNTERS = ("SQ_INSTS_VALU_FLOPS_FP32", "SQ_INSTS_VALU_FLOPS_FP64", "SQ_INSTS_VALU_MFMA_MOPS_F16",
+                    "VALUBusy", "MfmaUtil")
+PEAK_FP32_NONPACKED = 78.6e12  # flop/s: FP32 VALU without packed math (this build's kernels)
+PEAK_FP64_VALU = 78.6e12
+PEAK_MFMA_F16 = 2.5e15         # dense
+
+
+def profile_compute_run(args):
+    """Single dispatches of the compute-bound kernels for rocprofv3: the
+    pair-by-pair forward at 2^24 x 256 (FP32 VALU) and the split-FP16 batched
+    contraction at 65536 x 512 x 512, three and one products (MFMA)."""
+    import torch
+
+    from ragnar_amd import torch_ops as T
+    from ragnar_amd import workloads
+    from oracle import oracle as orc
+
+    n, nb = args.particles, args.bins
+    u_h, b_h = workloads.particles(n)
+    U = torch.from_numpy(u_h).cuda()
+    B = torch.from_numpy(b_h).cuda()
+    plan = T.SyncPlan(T.default_table(), orc.logspace(1e-2, 1e7, nb))
+    S = torch.empty(nb, device="cuda")
+    nd, ng, nbc = 65536, 512, 512
+    f, _, _ = workloads.distributions(nd, ng)
+    G = torch.from_numpy(orc.logspace(1, 1e6, ng)).cuda()
+    Fm = torch.from_numpy(f).cuda()
+    cplan = T.SyncPlan(T.default_table(), orc.logspace(1e-2, 1e7, nbc))
+    out = torch.empty(nd, nbc, device="cuda")
+    reps = args.profile_reps
+    for _ in range(args.warmup):
+        T.sync_spectrum(plan, U, B, out=S, method="direct")
+        T.sync_fromdist_batched(cplan, G, Fm, True, 1, out=out)
+    torch.cuda.synchronize()
+    print(f"profile: forward (direct) x{reps}", flush=True)
+    for _ in range(reps):
+        T.sync_spectrum(plan, U, B, out=S, method="direct")
+    for prec, name in ((1, "f16x3"), (2, "f16x1")):
+        print(f"profile: contraction {name} x{reps}", flush=True)
+        for _ in range(reps):
+            T.sync_fromdist_batched(cplan, G, Fm, True, prec, out=out)
+    torch.cuda.synchronize()
+    print("profile: done", flush=True)
+
+
+def summarize_compute(trace_dir, pmc_dir):
+    """Achieved FP32 VALU, FP64 VALU and F16 MFMA rates of the --profile-compute
+    kernels against MI355X peaks, from rocprofv3 CSV output: counter values
+    per dispatch (warm dispatches averaged, split by template instance via the
+    dispatch order of the trace) over the kernel's average traced duration."""
+    import csv
+    import statistics
+
+    def demangle(name):
+        """Kernel name and integer / bool template arguments of a mangled
+        _ZN...E name (what these kernels need; c++filt here cannot parse the
+        _Float16 parameter types), else the name unchanged."""
+        import re
+
+        if not name.startswith("_ZN"):
+            return name
+        i, parts = 3, []
+        while i < len(name) and name[i].isdigit():
+            m = re.match(r"\d+", name[i:])
+            n = int(m.group())
+            i += len(m.group())
+            parts.append(name[i:i + n])
+            i += n
+        args = []
+        if i < len(name) and name[i] == "I":
+            for kind, val in re.findall(r"L([ib])(\d+)E", name[i:name.find("EE", i) + 1]):
+                args.append(("true" if val == "1" else "false") if kind == "b" else val)
+        base = "::".join(p for p in parts if not p.startswith("_GLOBAL__N"))
+        return "void " + base + (f"<{', '.join(args)}>" if args else "") + "()"
+
+    trace = list(csv.DictReader(open(Path(trace_dir) / "run_kernel_trace.csv")))
+    dur = {}
+    for r in trace:
+        dur.setdefault(r["Kernel_Name"], []).append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-9)
+    vals = {}
+    for r in csv.DictReader(open(Path(pmc_dir) / "run_counter_collection.csv")):
+        vals.setdefault(r["Kernel_Name"], {}).setdefault(r["Counter_Name"], {}).setdefault(
+            int(r["Dispatch_Id"]), 0.0)
+        vals[r["Kernel_Name"]][r["Counter_Name"]][int(r["Dispatch_Id"])] += float(r["Counter_Value"])
+
+    def warm(v):
+        v = [x for _, x in sorted(v.items())]
+        return statistics.mean(v[1:] if len(v) > 1 else v)
+
+    rows = ["| kernel | dispatches | avg duration (us) | FP32 VALU TFLOP/s | FP64 VALU TFLOP/s | "
+            "F16 MFMA TFLOP/s | VALUBusy % | MfmaUtil % |", "|---|---|---|---|---|---|---|---|"]
+    report = {}
+    for k, d in sorted(dur.items(), key=lambda kv: -statistics.mean(kv[1])):
+        full = demangle(k)
+        short = _short_name(full)
+        if short not in ("forward_kernel", "combine_kernel", "mfma_contract", "pack_kernel_matrix"):
+            continue
+        c = vals.get(k, {})
+        t = statistics.mean(d[1:] if len(d) > 1 else d)
+        # the SQ VALU instruction counters count wave instructions: x 64 lanes
+        # (as rocprofv3's own TOTAL_32_OPS does)
+        f32 = (warm(c["SQ_INSTS_VALU_FLOPS_FP32"]) * 64 / t / 1e12
+               if "SQ_INSTS_VALU_FLOPS_FP32" in c else float("nan"))
+        f64 = (warm(c["SQ_INSTS_VALU_FLOPS_FP64"]) * 64 / t / 1e12
+               if "SQ_INSTS_VALU_FLOPS_FP64" in c else float("nan"))
+        mf = (warm(c["SQ_INSTS_VALU_MFMA_MOPS_F16"]) * 512 / t / 1e12
+              if "SQ_INSTS_VALU_MFMA_MOPS_F16" in c else float("nan"))
+        vb = warm(c["VALUBusy"]) if "VALUBusy" in c else float("nan")
+        mu = warm(c["MfmaUtil"]) if "MfmaUtil" in c else float("nan")
+        base = full.replace("(anonymous namespace)::", "").split("(")[0].replace("void ", "").strip()
+        tmpl = short + (base[base.index("<"):] if "<" in base else "")
+        tmpl = tmpl.replace("rgn::", "")
+        rows.append(f"| `{tmpl}` | {len(d)} | {t * 1e6:.1f} | {f32:.1f} | {f64:.1f} | {mf:.0f} | {vb:.0f} | {mu:.0f} |")
+        report[tmpl] = {"us": t * 1e6, "fp32_tflops": f32, "fp64_tflops": f64, "mfma_f16_tflops": mf,
+                        "valu_busy_pct": vb, "mfma_util_pct": mu}
+    md = ["# VALU and MFMA rates of the compute-bound kernels (MI355X)", "",
+          "Made by `python bench.py --summarize-compute <trace> <pmc>` from two rocprofv3 runs of",
+          "`bench.py --profile-compute` (commands in profiles/README.md): the pair-by-pair",
+          "forward at 2^24 particles x 256 bins and the split-FP16 contraction at 65536 x 512 x 512",
+          "(`mfma_contract<3, true>`: three products; `<1, true>`: one).", "",
+          f"Peaks: FP32 VALU {PEAK_FP32_NONPACKED / 1e12:.1f} TFLOP/s without packed math "
+          f"({PEAK_FP32 / 1e12:.1f} with it; this build compiles with packed FP32 off), "
+          f"FP64 VALU {PEAK_FP64_VALU / 1e12:.1f}, dense F16 MFMA {PEAK_MFMA_F16 / 1e12:.0f}.",
+          "Counters: " + ", ".join(f"`{c}`" for c in COMPUTE_COUNTERS) + " (one --pmc pass).  The "
+          "VALU flop counters count per wave instruction and are scaled by 64 lanes; MFMA flops = "
+          "MOPS_F16 x 512; VALUBusy and MfmaUtil are rocprofv3's derived percentages (VALUBusy "
+          "sums four SIMDs per CU, so it can exceed 100).", "",
+          *rows, ""]
+    (ROOT / "profiles" / "compute_units.md").write_text("\n".join(md))
+    print("\n".join(md))
+    return report
 
 
 def _short_name(name):
@@ -368,8 +510,14 @@ def main():
     if args.summarize:
         summarize(*args.summarize, args.method)
         return
+    if args.summarize_compute:
+        summarize_compute(*args.summarize_compute)
+        return
     if args.profile:
         profile_run(args)
+        return
+    if args.profile_compute:
+        profile_compute_run(args)
         return
 
     import numpy as np
