@@ -211,7 +211,8 @@ __device__ __forceinline__ T block_prefix(T v, T& total, T* wsum) {
 }
 
 // Block-wide exclusive scan of a[0..N) into o[0..N) (o may alias a); returns
-// the total.  part: NT words of LDS.  Every thread of the NT-thread block calls it.
+// the total.  part: NT / 64 words of LDS.  Every thread of the NT-thread block
+// calls it.
 template <class T, int NT = kThreads, class PA, class PO>
 __device__ T block_scan_exclusive(PA a, PO o, int N, T* part) {
   const int per = (N + NT - 1) / NT;
@@ -254,7 +255,7 @@ __global__ void __launch_bounds__(kBucketThreads) binned_bucket(ParticlesDev D, 
   unsigned* hist = reinterpret_cast<unsigned*>(smem);
   unsigned* off = hist + NK;
   unsigned* part = off + NK;
-  float2* sWP = reinterpret_cast<float2*>(part + kBucketThreads);  // after 2 NK + kBucketThreads words: 8-byte aligned
+  float2* sWP = reinterpret_cast<float2*>(part + kBucketThreads / 64);  // after an even word count: 8-byte aligned
   for (int i = threadIdx.x; i < NK; i += kBucketThreads) hist[i] = 0u;
   __syncthreads();
   const unsigned G = gridDim.x;
@@ -577,7 +578,7 @@ __global__ void __launch_bounds__(kPrefThreads) binned_prefix(int M, const unsig
                                                               const double* __restrict__ partial,
                                                               double* __restrict__ P0, double* __restrict__ P1) {
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ double part[kPrefThreads];
+  __shared__ double part[kPrefThreads / 64];
   __shared__ double red[2][4][256];
   const int M1 = M + 1;
   double* s0 = reinterpret_cast<double*>(smem);
@@ -684,7 +685,7 @@ __global__ void __launch_bounds__(kThreads) binned_adj_table(BinnedDev P, const 
   extern __shared__ __align__(16) unsigned char smem[];
   double* d0 = reinterpret_cast<double*>(smem);
   double* d1 = d0 + P.M;
-  __shared__ double part[kThreads];
+  __shared__ double part[kThreads / 64];
   __shared__ double base0[kThreads], base1[kThreads];
   const int kk = blockIdx.x;
   const int k = P.kmin + kk;
@@ -787,7 +788,7 @@ __global__ void __launch_bounds__(kThreads) binned_adj_gather(ParticlesDev D, si
 // LDS bytes of the bucket pass for a plan (hist, offsets, scan partials and
 // the block's sorted sources).
 static size_t bucket_smem(const SyncPlan& plan) {
-  return sizeof(unsigned) * (2 * static_cast<size_t>(plan.NK) + kBucketThreads) + 2 * sizeof(float) * kChunk;
+  return sizeof(unsigned) * (2 * static_cast<size_t>(plan.NK) + kBucketThreads / 64) + 2 * sizeof(float) * kChunk;
 }
 
 static size_t subhist_smem(const SyncPlan& plan) {
