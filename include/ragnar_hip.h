@@ -115,7 +115,7 @@ int rgn_sync_adjoint(const rgn_sync_plan* plan, const float* const d_u[3],
  * by the integer part of their table coordinate and reduced into per-sub-cell
  * moments (forward), or read one precomputed entry each (adjoint); cost is
  * ~independent of the bin count.  RGN_METHOD_AUTO picks the cheaper one from a
- * measured cost model (forward: binned from ~1.5e6 particles at 256 bins; env
+ * measured cost model (forward: binned from ~1.2e6 particles at 256 bins; env
  * RAGNAR_SYNC_METHOD=direct|binned overrides).  Binned needs fewer than 2^29
  * particles, at most 1024 distinct fractional bin offsets and a bin range of
  * at most 3072 table units (RGN_ERR_UNSUPPORTED otherwise; AUTO then runs

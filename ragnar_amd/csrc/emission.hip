@@ -515,11 +515,18 @@ int resolve_method(const SyncPlan& plan, size_t n, int method, bool adjoint) {
       if (std::string(m) == "binned" && binned_fits) return RGN_METHOD_BINNED;
     }
     if (!binned_fits) return RGN_METHOD_DIRECT;
-    static const double c_nb[9] = {0, 22e-12, 29e-12, 35e-12, 41e-12, 55e-12, 70e-12, 82e-12, 95e-12};
+    // Seconds per call, fitted on MI355X to 2^16 .. 2^24 particles at 32 .. 512
+    // bins.  The pair-by-pair kernels cost per particle by NB, the number of
+    // 64-bin blocks (seconds per particle below, by NB); the factorised ones
+    // have a floor of fixed passes, then a cost per particle that hardly
+    // depends on the bin count.
+    static const double c_fwd[9] = {0, 20e-12, 26e-12, 31e-12, 36e-12, 50e-12, 65e-12, 80e-12, 94e-12};
+    static const double c_adj[9] = {0, 27e-12, 34e-12, 43e-12, 52e-12, 63e-12, 74e-12, 84e-12, 95e-12};
     const double nd = static_cast<double>(n);
-    const double t_direct = adjoint ? 25e-6 + nd * plan.nbins / 4.6e12
-                                    : 18e-6 + nd * c_nb[std::min(std::max(plan.NB, 1), 8)];
-    const double t_binned = adjoint ? 28e-6 + nd * 14e-12 : 60e-6 + nd * 13.5e-12;
+    const int nbk = std::min(std::max(plan.NB, 1), 8);
+    const double t_direct = adjoint ? 25e-6 + nd * c_adj[nbk] : 18e-6 + nd * c_fwd[nbk];
+    const double t_binned = adjoint ? std::max(33e-6, 28e-6 + nd * 13.5e-12)
+                                    : std::max(62e-6, 35e-6 + nd * 10.1e-12);
     return t_binned < t_direct ? RGN_METHOD_BINNED : RGN_METHOD_DIRECT;
   }
   RGN_REQUIRE(method == RGN_METHOD_DIRECT || method == RGN_METHOD_BINNED, RGN_ERR_INVALID,
