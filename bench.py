@@ -624,6 +624,14 @@ def main():
         G, Fd = torch.from_numpy(g).to(dev), torch.from_numpy(f).to(dev)
         ms_d = event_time(lambda: T.sync_fromdist(dplan, G, Fd, True), 50, 5, torch)
         extras["fromdist_200x200"] = {"us": ms_d * 1e3}
+        # inverse-Compton spectrum, the reference test's 200 x 200 x 200 triples
+        gi = orc.logspace(1e3, 1e7, 200)
+        si = orc.logspace(1e-11, 1e-7, 200)
+        ti = [torch.from_numpy(a).to(dev) for a in (gi, orc.generator("plaw", [-1.5, 1e3, 1e7], gi), si,
+                                                     orc.generator("delta", [1e-8, 1e-9], si),
+                                                     orc.logspace(1e3, 1e7, 200))]
+        ms_i = event_time(lambda: T.ic_spectrum(ti[0], ti[1], True, ti[2], ti[3], ti[4]), 50, 5, torch)
+        extras["ic_200x200x200"] = {"us": ms_i * 1e3, "triples_per_s": 200 ** 3 / (ms_i * 1e-3)}
         for name, fn in (("cells_4096", lambda: bench_cells(T, workloads, orc, torch, table)),
                          ("contraction_65536x512x512",
                           lambda: bench_contraction(T, workloads, orc, torch, table))):

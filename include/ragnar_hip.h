@@ -200,6 +200,19 @@ int rgn_hist_plan_destroy(rgn_hist_plan* plan);
 int rgn_energy_hist(const rgn_hist_plan* plan, const float* const d_u[3], size_t n,
                     int fourvel, float* d_hist, uint64_t* d_counts, rgn_stream_t stream);
 
+/* ---- inverse-Compton spectrum ------------------------------------------------ */
+/* ICSpectrum (src/physics/ic.cpp:15-46, per-triple kernel src/physics/ic.hpp:28-86):
+ * E_ic^2 f(E_ic) at the neic energies d_eic (mc^2 units) of particles with
+ * gamma bins d_g and values d_fg (ng; g_log: the bins are log spaced, which
+ * divides by g instead of g^2) scattering soft photons of energies d_es (mc^2)
+ * and values d_fes (nes).  Terms in the reference's FP32 arithmetic, summed per
+ * output bin in FP64 in a fixed order.  The unit checks of the reference's
+ * kernel constructor (ic.hpp:52-59) belong to the caller (the pybind layer
+ * raises them as the reference does). */
+int rgn_ic_spectrum(const float* d_g, const float* d_fg, int ng, int g_log, const float* d_es,
+                    const float* d_fes, int nes, const float* d_eic, int neic, float* d_out,
+                    rgn_stream_t stream);
+
 /* ---- device buffers (used by the Python layer) ---------------------------- */
 int rgn_malloc(void** d_ptr, size_t bytes);
 int rgn_free(void* d_ptr);

@@ -56,6 +56,8 @@ def lib():
         L.orc_sync_fromdist.argtypes = [C.c_int, _f32, _f32, C.c_size_t, C.c_int, _f32,
                                         C.c_size_t, _f32, _f32, C.c_size_t, C.c_float,
                                         C.c_float, _f32, vp]
+        L.orc_ic_spectrum.argtypes = [C.c_int, _f32, _f32, C.c_size_t, C.c_int, _f32, _f32,
+                                      C.c_size_t, _f32, C.c_size_t, _f32, vp]
         L.orc_energy_hist.argtypes = [C.c_int, _f32, _f32, _f32, C.c_size_t, _f32, C.c_size_t,
                                       C.c_int, C.c_int, _f32, vp, vp]
         L.orc_generator.restype = C.c_int
@@ -160,6 +162,20 @@ def sync_fromdist(gbeta, f, dist_log, bins, g_syn=1.0, e_at=1.0, mode=EXACT, tab
     lib().orc_sync_fromdist(mode, gbeta, f, len(gbeta), int(dist_log), bins, len(bins),
                             np.ascontiguousarray(fx, np.float32), np.ascontiguousarray(fy, np.float32),
                             len(fx), g_syn, e_at, out, outd.ctypes.data)
+    return (out, outd) if want_fp64 else out
+
+
+def ic_spectrum(g, fg, g_log, es, fes, eic, mode=EXACT, want_fp64=False):
+    """ICSpectrum (ic.cpp:15-46): E_ic^2 f(E_ic) at eic; (float32, FP64 sums)."""
+    g = np.ascontiguousarray(g, np.float32)
+    fg = np.ascontiguousarray(fg, np.float32)
+    es = np.ascontiguousarray(es, np.float32)
+    fes = np.ascontiguousarray(fes, np.float32)
+    eic = np.ascontiguousarray(eic, np.float32)
+    out = np.zeros(len(eic), np.float32)
+    outd = np.zeros(len(eic), np.float64)
+    lib().orc_ic_spectrum(mode, g, fg, len(g), int(g_log), es, fes, len(es), eic, len(eic), out,
+                          outd.ctypes.data)
     return (out, outd) if want_fp64 else out
 
 

@@ -54,6 +54,7 @@ _SIGNATURES = {
     "rgn_sync_adjoint_stages": (C.c_int, [vp, fptr3, fptr3, fptr3, vp, sz, vp, C.c_int, vp, fptr3,
                                           fptr3, fptr3, vp, vp, vp, C.c_int, C.POINTER(C.c_int)]),
     "rgn_sync_fromdist": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, vp, vp]),
+    "rgn_ic_spectrum": (C.c_int, [vp, vp, C.c_int, C.c_int, vp, vp, C.c_int, vp, C.c_int, vp, vp]),
     "rgn_sync_fromdist_batched": (C.c_int, [vp, vp, C.c_int, C.c_int, vp, sz, C.c_int, vp, vp]),
     "rgn_hist_plan_create": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(vp)]),
     "rgn_hist_plan_destroy": (C.c_int, [vp]),

@@ -26,7 +26,7 @@ BUILD = HERE / "_build"
 INCLUDE = ROOT / "include"
 
 HIP_SOURCES = ["rgn_core.hip", "ftable.hip", "emission.hip", "binned.hip", "cells.hip",
-               "histogram.hip", "contraction.hip", "contraction_f16.hip"]
+               "histogram.hip", "contraction.hip", "contraction_f16.hip", "ic.hip"]
 HOST_SOURCES = ["host/pyragnar.cpp", "host/containers.cpp", "host/h5io.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

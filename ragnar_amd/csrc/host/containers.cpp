@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <mutex>
 #include <stdexcept>
 
@@ -604,6 +605,35 @@ Array1D<real_t> SynchrotronSpectrumFromDist(const TabulatedDistribution& dist, c
                             out.dptr() + c0, nullptr));
     check(rgn_sync(nullptr));
   }
+  out.mark_device_written();
+  if (verbose()) py::print(": OK", "flush"_a = true);
+  return out;
+}
+
+// ic.cpp:15-46.  The progress lines come first and the unit checks of the
+// reference's kernel constructor (ic.hpp:52-59) after them, as in the reference.
+Array1D<real_t> ICSpectrum(const TabulatedDistribution& dist_prtls,
+                           const TabulatedDistribution& dist_soft_photons, const Bins& bins_e_ic) {
+  if (verbose()) py::print("Computing IC spectrum ...", "flush"_a = true);
+  const std::size_t ng = dist_prtls.extent(), nes = dist_soft_photons.extent(), nic = bins_e_ic.size();
+  if (verbose()) {
+    py::print(" Launching", ToHumanReadable(static_cast<double>(ng * nes * nic), false), "threads",
+              "end"_a = "", "flush"_a = true);
+  }
+  if (dist_soft_photons.EnergyBins().unit != EnergyUnits::mec2) {
+    throw std::runtime_error("Soft photons energy bins must be in units of mec^2");
+  }
+  if (bins_e_ic.unit != EnergyUnits::mec2) {
+    throw std::runtime_error("E_ic must be in units of mec^2");
+  }
+  constexpr std::size_t kMaxBins = static_cast<std::size_t>(std::numeric_limits<int>::max());
+  if (ng > kMaxBins || nes > kMaxBins || nic > kMaxBins) throw std::range_error("too many bins");
+  Array1D<real_t> out = Array1D<real_t>::device(nic);
+  check(rgn_ic_spectrum(dist_prtls.EnergyBins().dptr(), dist_prtls.F().dptr(), static_cast<int>(ng),
+                        dist_prtls.log_spaced() ? 1 : 0, dist_soft_photons.EnergyBins().dptr(),
+                        dist_soft_photons.F().dptr(), static_cast<int>(nes), bins_e_ic.dptr(),
+                        static_cast<int>(nic), out.dptr(), nullptr));
+  check(rgn_sync(nullptr));
   out.mark_device_written();
   if (verbose()) py::print(": OK", "flush"_a = true);
   return out;

@@ -185,6 +185,9 @@ struct Particles {
 real_t Ffunc_integrand(real_t x);
 Array1D<real_t> SynchrotronSpectrumFromDist(const TabulatedDistribution& dist, const Bins& bins,
                                             real_t g_syn, real_t e_syn_at_g_syn);
+// ic.cpp:15-46
+Array1D<real_t> ICSpectrum(const TabulatedDistribution& dist_prtls,
+                           const TabulatedDistribution& dist_soft_photons, const Bins& bins_e_ic);
 template <dim_t D>
 Array1D<real_t> SynchrotronSpectrum(const Particles<D>& prtls, const Bins& bins, real_t B0,
                                     real_t g_syn, real_t e_syn_at_g_syn);

@@ -210,13 +210,9 @@ PYBIND11_MODULE(ragnar, m) {
   define_sync<2>(m);
   define_sync<3>(m);
 
-  // ic.cpp:48-74 (inverse-Compton: not part of this backend's hot path)
-  m.def(
-    "ICSpectrum",
-    [](const TabulatedDistribution&, const TabulatedDistribution&, const Bins&) -> Array1D<real_t> {
-      throw std::runtime_error("ICSpectrum is not implemented on the MI355X backend");
-    },
-    "dist_prtls"_a, "dist_soft_photons"_a, "bins_e_ic"_a);
+  // ic.cpp:48-74
+  m.def("ICSpectrum", &ICSpectrum, "dist_prtls"_a, "dist_soft_photons"_a, "bins_e_ic"_a,
+        "Inverse-Compton spectrum E^2 dN/dE of a particle distribution scattering soft photons");
 
   m.def("_default_ftable_nodes", []() {
     const rgn_ftable* t = default_ftable();

@@ -161,6 +161,19 @@ def sync_fromdist(plan: SyncPlan, gbeta, f, dist_log: bool, out=None) -> torch.T
     return out
 
 
+def ic_spectrum(g, fg, g_log: bool, es, fes, eic, out=None) -> torch.Tensor:
+    """ICSpectrum (src/physics/ic.cpp:15-46) on device tensors: E_ic^2 f(E_ic)
+    at the energies eic (mc^2) of particles (gamma bins g, values fg; g_log:
+    log-spaced bins) scattering soft photons (energies es in mc^2, values fes)."""
+    ng, nes, neic = g.shape[0], es.shape[0], eic.shape[0]
+    if out is None:
+        out = torch.empty(neic, dtype=torch.float32, device=eic.device)
+    check(lib().rgn_ic_spectrum(_vec(g, "g", ng), _vec(fg, "fg", ng), ng, int(g_log), _vec(es, "es", nes),
+                                _vec(fes, "fes", nes), nes, _vec(eic, "eic", neic), neic, out.data_ptr(),
+                                _stream()))
+    return out
+
+
 def sync_fromdist_batched(plan: SyncPlan, gbeta, f2d, dist_log: bool, precision: int,
                           out=None) -> torch.Tensor:
     ng = gbeta.shape[0]

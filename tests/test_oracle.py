@@ -315,3 +315,94 @@ def test_generators_normalisation(orc):
         e = grid(1e-2, 1, int(1e6))
         assert np.isclose(np.trapezoid(orc.generator("plaw", [-1.2, 1e-2, 1], e), e), 1)
     assert orc.generator("plaw", [-2.0, 0.0, 0.0], np.ones(1)) is None  # diverges
+
+
+# ---- ICSpectrum (ic.cpp:15-46, ic.hpp:20-85) --------------------------------
+# Pins, none of which retypes the kernel:
+#  - the energy-loss rate of one electron on mono-energetic photons, whose
+#    textbook limits are (4/3) sigma_T c gamma^2 U (Thomson) and
+#    ln(4 gamma eps) - 11/6 (Klein-Nishina), i.e. int E KN dE / g^2 -> Gamma^2 / 9
+#    and (ln Gamma - 11/6) / 2 for Gamma = 4 g eps (Blumenthal & Gould 1970);
+#  - the kinematic cutoff E <= g Gamma / (1 + Gamma);
+#  - the reference's own test (src/tests/ic.py:14-50): power-law slope and peak;
+#  - exact scaling in the distributions and the log/linear bin conventions.
+
+def _ic_one_electron(orc, g, eps, E, mode):
+    """E^2 f(E) of one electron (linear gamma bins, f = 1) on one photon line."""
+    return orc.ic_spectrum([g], [1.0], False, [eps], [1.0], E, mode=mode, want_fp64=True)[1]
+
+
+def _loss_integral(E, out):
+    """int out / E dE (trapezoid on the sorted grid)."""
+    return float(np.trapezoid(out / E, E))
+
+
+def test_ic_energy_loss_thomson_and_kn_limits(orc):
+    # Thomson: Gamma = 4e-6; the grid is logarithmic up to the cutoff
+    g, eps = 100.0, 1e-8
+    G = 4 * g * eps
+    emax = g * G / (1 + G)
+    E = np.unique(np.float32(np.geomspace(emax * 1e-7, emax * (1 - 1e-6), 20000)))
+    out = _ic_one_electron(orc, g, eps, E, orc.FP64)
+    val = _loss_integral(E.astype(np.float64), out) * eps  # int E KN dE / g^2
+    assert abs(val / (G * G / 9) - 1) < 1e-3, val / (G * G / 9)
+    # Klein-Nishina: Gamma = 4e4; resolve 1 - E/g logarithmically near the cutoff
+    g, eps = 1e3, 10.0
+    G = 4 * g * eps
+    t = np.geomspace(1.0 / (G + 1) * 1.0001, 0.5, 4000)
+    s = np.geomspace(1e-7, 0.5, 2000)
+    E = np.unique(np.concatenate([np.float32(g * (1 - t)), np.float32(g * s)]))
+    E = E[E <= np.float32(g * G / (1 + G))]
+    out = _ic_one_electron(orc, g, eps, E, orc.FP64)
+    val = _loss_integral(E.astype(np.float64), out) * eps
+    asym = (np.log(G) - 11.0 / 6.0) / 2
+    assert abs(val / asym - 1) < 1e-3, val / asym
+
+
+def test_ic_cutoff_scaling_and_modes(orc):
+    g = orc.logspace(10, 1e4, 60)
+    fg = orc.generator("plaw", (-2.0, 10, 1e4), g)
+    es = orc.logspace(1e-6, 1e-2, 40)
+    fes = orc.generator("plaw", (-1.0, 1e-6, 1e-2), es)
+    E = orc.logspace(1e-3, 2e4, 150)
+    ex, exd = orc.ic_spectrum(g, fg, True, es, fes, E, want_fp64=True)
+    # nothing above the largest kinematic limit, something below it
+    gmax, smax = np.float32(g.max()), np.float32(es.max())
+    Gm = np.float32(4) * gmax * smax
+    lim = gmax * Gm / (np.float32(1) + Gm)
+    assert np.all(ex[E > lim] == 0) and np.all(ex[E < 0.5 * lim] > 0)
+    # exact scaling: powers of two in f_g or f_s scale every float term exactly
+    ex2 = orc.ic_spectrum(g, 2 * fg, True, es, fes, E)
+    ex4 = orc.ic_spectrum(g, fg, True, es, 4 * fes, E)
+    assert np.array_equal(ex2, 2 * ex) and np.array_equal(ex4, 4 * ex)
+    # log bins divide by g, linear ones by g^2 (one gamma bin: ratio g)
+    one_log = orc.ic_spectrum([300.0], [1.0], True, es, fes, E)
+    one_lin = orc.ic_spectrum([300.0], [1.0], False, es, fes, E)
+    live = one_lin > 0
+    assert np.max(np.abs(one_log[live] / (300.0 * one_lin[live]) - 1)) < 1e-6
+    # float accumulators, FP64 sums of float terms and all-FP64 agree
+    fa = orc.ic_spectrum(g, fg, True, es, fes, E, mode=orc.FAITHFUL)
+    _, f64 = orc.ic_spectrum(g, fg, True, es, fes, E, mode=orc.FP64, want_fp64=True)
+    live = exd > 1e-6 * exd.max()
+    assert np.max(np.abs(fa[live] / exd[live] - 1)) < 1e-5
+    assert np.max(np.abs(exd[live] / f64[live] - 1)) < 1e-4
+
+
+def _slope(xs, ys, xmin, xmax):
+    mask = (xs > xmin) & (xs < xmax)
+    return np.polyfit(np.log10(xs[mask]), np.log10(ys[mask]), 1)[0]
+
+
+def test_reference_ic_on_oracle(orc):
+    """src/tests/ic.py:14-50 on the oracle: slope -p/2 + 3/2 between 2e3 and 2e4,
+    peak at e_break * (1e7)^2 within 10%."""
+    p, e_break = 1.5, 1e-8
+    g = orc.logspace(1e3, 1e7, 200)
+    fg = orc.generator("plaw", (-p, 1e3, 1e7), g)
+    es = orc.logspace(1e-11, 1e-7, 200)
+    fes = orc.generator("delta", (e_break, e_break / 10), es)
+    assert abs(_slope(g, fg, 1e3, 1e7) - (-p)) < 1e-2
+    E = orc.logspace(1e3, 1e7, 200)
+    out = orc.ic_spectrum(g, fg, True, es, fes, E)
+    assert abs(_slope(E, out, 2e3, 2e4) - (-p / 2 + 3 / 2)) < 1e-2
+    assert abs(E[np.argmax(out)] / (e_break * 1e7 ** 2) - 1) < 0.1
