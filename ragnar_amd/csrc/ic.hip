@@ -10,10 +10,11 @@
 // double in the reference: its 0.5 literal promotes it), and the terms of a bin
 // are summed in FP64 in a fixed order, so results are reproducible bit for bit.
 //
-// Layout: one workgroup per output bin; its threads stride over the
-// ng x nes pairs (the two distributions are a few KB and stay in cache), then
-// a fixed-order tree reduction.  The pair count per bin is what the reference
-// launches per bin; there is no table or plan.
+// Layout: one workgroup per (output bin, chunk of gamma bins), enough of them
+// to fill the device; the threads take the soft-photon bins (the two
+// distributions are a few KB and stay in cache); a fixed-order tree per block
+// and a second pass adding the chunks in order.  No table or plan.
+#include <algorithm>
 #include <cmath>
 
 #include "rgn_core.hpp"
@@ -33,27 +34,32 @@ __device__ __forceinline__ float kn_func(float Gamma, float q) {
   return static_cast<float>(static_cast<double>(first) + rest);
 }
 
+// One workgroup per (output bin, chunk of gamma bins): its threads take the
+// soft-photon bins of each gamma bin of the chunk, in order; the block's FP64
+// partial goes to partial[chunk][bin].
 __global__ void __launch_bounds__(kIcThreads) ic_kernel(const float* __restrict__ g, const float* __restrict__ fg,
                                                         int ng, int g_log, const float* __restrict__ es,
                                                         const float* __restrict__ fes, int nes,
-                                                        const float* __restrict__ eic, float* __restrict__ out) {
+                                                        const float* __restrict__ eic, int gper,
+                                                        double* __restrict__ partial) {
   __shared__ double red[kIcThreads];
-  const int e = blockIdx.x;
+  const int e = blockIdx.x, neic = gridDim.x;
   const float e_ic = eic[e];
-  const long long npair = static_cast<long long>(ng) * nes;
+  const int g0 = blockIdx.y * gper, g1 = min(ng, g0 + gper);
   double acc = 0.0;
-  for (long long p = threadIdx.x; p < npair; p += kIcThreads) {
-    const int gi = static_cast<int>(p / nes), si = static_cast<int>(p - static_cast<long long>(gi) * nes);
-    // ic.hpp:62-85, float arithmetic in the reference's order
+  for (int gi = g0; gi < g1; ++gi) {
     const float g_prtls = g[gi], f_prtls = fg[gi];
-    const float e_soft = es[si], f_soft = fes[si];
-    const float Gamma = 4.0f * g_prtls * e_soft;
-    if (e_ic > g_prtls * Gamma / (1.0f + Gamma)) continue;
-    const float q = (e_ic / g_prtls) / (Gamma * (1.0f - (e_ic / g_prtls)));
-    const float kn = kn_func(Gamma, q);
-    const float term = g_log ? f_prtls * (f_soft / e_soft) * e_ic * e_ic * kn / g_prtls
-                             : f_prtls * (f_soft / e_soft) * e_ic * e_ic * kn / (g_prtls * g_prtls);
-    acc += static_cast<double>(term);
+    for (int si = threadIdx.x; si < nes; si += kIcThreads) {
+      // ic.hpp:62-85, float arithmetic in the reference's order
+      const float e_soft = es[si], f_soft = fes[si];
+      const float Gamma = 4.0f * g_prtls * e_soft;
+      if (e_ic > g_prtls * Gamma / (1.0f + Gamma)) continue;
+      const float q = (e_ic / g_prtls) / (Gamma * (1.0f - (e_ic / g_prtls)));
+      const float kn = kn_func(Gamma, q);
+      const float term = g_log ? f_prtls * (f_soft / e_soft) * e_ic * e_ic * kn / g_prtls
+                               : f_prtls * (f_soft / e_soft) * e_ic * e_ic * kn / (g_prtls * g_prtls);
+      acc += static_cast<double>(term);
+    }
   }
   red[threadIdx.x] = acc;
   __syncthreads();
@@ -61,7 +67,17 @@ __global__ void __launch_bounds__(kIcThreads) ic_kernel(const float* __restrict_
     if (static_cast<int>(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[e] = static_cast<float>(red[0]);
+  if (threadIdx.x == 0) partial[static_cast<size_t>(blockIdx.y) * neic + e] = red[0];
+}
+
+// out[e] = the chunk partials of bin e, summed in chunk order.
+__global__ void __launch_bounds__(kIcThreads) ic_combine(const double* __restrict__ partial, int nchunk, int neic,
+                                                         float* __restrict__ out) {
+  const int e = blockIdx.x * kIcThreads + threadIdx.x;
+  if (e >= neic) return;
+  double s = 0.0;
+  for (int c = 0; c < nchunk; ++c) s += partial[static_cast<size_t>(c) * neic + e];
+  out[e] = static_cast<float>(s);
 }
 
 }  // namespace
@@ -81,9 +97,18 @@ int rgn_ic_spectrum(const float* d_g, const float* d_fg, int ng, int g_log, cons
     RGN_REQUIRE((ng == 0 || (d_g && d_fg)) && (nes == 0 || (d_es && d_fes)), RGN_ERR_INVALID,
                 "null argument");
     hipStream_t st = as_stream(stream);
-    ic_kernel<<<neic, kIcThreads, 0, st>>>(d_g, d_fg, ng, g_log ? 1 : 0, d_es, d_fes, nes, d_eic, d_out);
+    // enough workgroups to fill the device: gamma bins split into chunks
+    const int chunks = std::min(std::max(1, (8 * num_cus() + neic - 1) / neic), std::max(ng, 1));
+    const int gper = std::max(1, (ng + chunks - 1) / chunks);
+    const int nchunk = std::max(1, (ng + gper - 1) / gper);  // at most chunks <= 8 CUs + 1
+    double* partial = static_cast<double*>(scratch(sizeof(double) * nchunk * static_cast<size_t>(neic), 17));
+    ic_kernel<<<dim3(neic, nchunk), kIcThreads, 0, st>>>(d_g, d_fg, ng, g_log ? 1 : 0, d_es, d_fes, nes, d_eic,
+                                                         gper, partial);
     RGN_HIP(hipGetLastError());
     stage_mark(st, "ic_kernel");
+    ic_combine<<<(neic + kIcThreads - 1) / kIcThreads, kIcThreads, 0, st>>>(partial, nchunk, neic, d_out);
+    RGN_HIP(hipGetLastError());
+    stage_mark(st, "ic_combine");
   });
 }
 
