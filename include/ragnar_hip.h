@@ -13,6 +13,10 @@
  * semantics (every reference compute call ends in Kokkos::fence()) call
  * rgn_sync(stream).
  *
+ * The library works on one device (rgn_init) and keeps one set of scratch
+ * buffers on it; compute calls that overlap on different streams share them
+ * and race, so callers order such calls (one stream, or events).
+ *
  * Reference interfaces replaced are cited as path:line in haykh/ragnar.
  */
 #ifndef RAGNAR_HIP_H

@@ -801,7 +801,12 @@ void sync_forward_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, 
   RGN_REQUIRE(plan.binned_ok && n < kBinnedMaxParticles, RGN_ERR_UNSUPPORTED, "plan outside the binned limits");
   const BinnedDev P = binned_view(plan);
   if (plan.NK == 0 || n == 0) {
-    RGN_HIP(hipMemsetAsync(out, 0, sizeof(float) * plan.nbins, st));
+    // a fill kernel rather than a memset node, so the call stays safe to capture
+    if (plan.nbins > 0) {
+      fill_zero<<<static_cast<unsigned>(ceil_div(plan.nbins, kThreads)), kThreads, 0, st>>>(
+        reinterpret_cast<unsigned*>(out), static_cast<size_t>(plan.nbins));
+      RGN_HIP(hipGetLastError());
+    }
     return;
   }
   const int G = static_cast<int>(ceil_div(n, kChunk));

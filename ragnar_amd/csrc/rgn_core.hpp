@@ -70,9 +70,10 @@ struct DeviceBuffer {
   T* as() const { return static_cast<T*>(ptr); }
 };
 
-// Per-device scratch reused across calls on the same stream order (grows on
-// demand; never shrinks).  Calls that overlap on different streams must not
-// share it; the library serialises scratch users through `scratch_lock`.
+// Scratch of the library's device, one buffer per slot, reused by every call
+// (grows on demand; never shrinks).  The mutex only guards the slot table:
+// kernels of calls that overlap on different streams share the buffers and
+// race, so such calls must be ordered by the caller.
 void* scratch(size_t bytes, int slot);
 // The same; *fresh is set when the slot was (re)allocated by this call, so its
 // previous contents are gone.
