@@ -169,8 +169,8 @@ __global__ void __launch_bounds__(kThreadsC) mfma_contract(const float* __restri
       }
       if (threadIdx.x < kTN) eB[threadIdx.x] = reB;
     }
-    __syncthreads();
     if (kb + 1 < nkb) load_block(kb + 1);  // in flight during the matrix work
+    __syncthreads();
     float16v tmp[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
