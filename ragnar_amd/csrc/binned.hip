@@ -307,7 +307,10 @@ __global__ void __launch_bounds__(kBucketThreads) binned_bucket(ParticlesDev D, 
 // Pass 2a: per bucket k, its non-empty runs in block order, compacted:
 //   cvoff[k * (G + 1) + i]: virtual offset of run i within the bucket
 //                           (cvoff[k * (G + 1) + nrun[k]] = totals[k]);
-//   cidx[k * G + i]:        physical index of the run's first source.
+//   cidx[k * G + i]:        physical index of the run's first source;
+//   irun[k * IS + t]:       the run holding source t * kItem of the bucket, the
+//                           first source of its work item t (a run is shorter
+//                           than kItem, so it holds at most one such source).
 // One exclusive scan over blocks of (run length, non-empty flag) packed in 64 bits.
 // The descriptors read are zeroed again, so the table needs no clearing pass
 // before the next call.
@@ -315,7 +318,8 @@ __global__ void __launch_bounds__(kRowThreads) binned_scan_rows(unsigned* __rest
                                                              unsigned* __restrict__ cvoff,
                                                              unsigned* __restrict__ cidx,
                                                              unsigned* __restrict__ nrun,
-                                                             unsigned* __restrict__ totals) {
+                                                             unsigned* __restrict__ totals,
+                                                             unsigned* __restrict__ irun, int IS) {
   __shared__ unsigned long long part[kRowThreads / 64];
   const int k = blockIdx.x;
   unsigned* row = runs + static_cast<size_t>(k) * G;
@@ -346,8 +350,11 @@ __global__ void __launch_bounds__(kRowThreads) binned_scan_rows(unsigned* __rest
     const unsigned len = run_len(r);
     if (len) {
       const unsigned c = static_cast<unsigned>(acc >> 32);
-      vo[c] = static_cast<unsigned>(acc);
+      const unsigned v = static_cast<unsigned>(acc);
+      vo[c] = v;
       ci[c] = static_cast<unsigned>(i) * kChunk + run_start(r);
+      const unsigned t = (v + kItem - 1) / kItem;  // first item start at or after v
+      if (t * kItem < v + len) irun[static_cast<size_t>(k) * IS + t] = c;
       acc += len + (1ull << 32);
       row[i] = 0u;  // leave the run table zeroed for the next call
     }
@@ -419,6 +426,7 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
                                                               const unsigned* __restrict__ cvoff,
                                                               const unsigned* __restrict__ cidx,
                                                               const unsigned* __restrict__ nrun,
+                                                              const unsigned* __restrict__ irun, int IS,
                                                               const float2* __restrict__ sortWP,
                                                               double* __restrict__ partial) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -431,12 +439,11 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
   float* athr = reinterpret_cast<float*>(lut + JQ2);
   unsigned* wro = reinterpret_cast<unsigned*>(athr + P.M);   // kWin + 1
   unsigned* wlo = wro + kWin + 1;                              // kWin
-  __shared__ unsigned s_cnt[2], s_item[2], s_wsum[kWaves];
+  __shared__ unsigned s_item[2], s_wsum[kWaves];
   const unsigned it = blockIdx.x;
   int k;
   unsigned first;
   if (!item_bucket<kSubThreads>(totals, P.NK, it, k, first, s_wsum, s_item)) return;  // uniform per block
-  if (threadIdx.x < 2) s_cnt[threadIdx.x] = 0u;
   for (int i = threadIdx.x; i < M1; i += kSubThreads) {
     m0[i] = 0.0;
     m1[i] = 0.0;
@@ -456,22 +463,18 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
   const unsigned v1 = min(totals[k], v0 + kItem);
   const unsigned* vo = cvoff + static_cast<size_t>(k) * (G + 1);
   const unsigned* ci = cidx + static_cast<size_t>(k) * G;
-  const int nr = static_cast<int>(nrun[k]);
-  // first and last run of the item, a monotone search done as a count (all
-  // loads in flight at once): #{i < nr : vo[i] <= v} - 1 (vo strictly
-  // increases: every listed run is non-empty)
-  {
-    unsigned c0 = 0, c1 = 0;
-    for (int q = threadIdx.x; q < nr; q += kSubThreads) {
-      const unsigned r = vo[q];
-      c0 += r <= v0;
-      c1 += r <= v1 - 1;
-    }
-    if (c0) atomicAdd(&s_cnt[0], c0);
-    if (c1) atomicAdd(&s_cnt[1], c1);
+  // first and last run of the item: the runs holding sources v0 and v1 - 1.
+  // The scan pass recorded the run holding each item's first source; the
+  // next item's first run holds v1 - 1 too unless it starts exactly at v1.
+  const unsigned* ir = irun + static_cast<size_t>(k) * IS + (it - first);
+  const int rlo = static_cast<int>(ir[0]);
+  int rhi;
+  if (v1 == totals[k]) {
+    rhi = static_cast<int>(nrun[k]) - 1;
+  } else {
+    const int r1 = static_cast<int>(ir[1]);
+    rhi = vo[r1] == v1 ? r1 - 1 : r1;
   }
-  __syncthreads();
-  const int rlo = static_cast<int>(s_cnt[0]) - 1, rhi = static_cast<int>(s_cnt[1]) - 1;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int ws = rlo; ws <= rhi; ws += kWin) {
     const int nc = min(rhi - ws + 1, kWin);
@@ -820,6 +823,9 @@ void sync_forward_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, 
   unsigned* cvoff = static_cast<unsigned*>(scratch(sizeof(unsigned) * NK * (G + 1), 10));
   unsigned* cidx = static_cast<unsigned*>(scratch(sizeof(unsigned) * NK * G, 9));
   unsigned* small = static_cast<unsigned*>(scratch(sizeof(unsigned) * (2 * (NK + 1) + 64), 11));
+  // first runs of the work items: a bucket holds at most ceil(G kChunk / kItem) items
+  const int IS = static_cast<int>(ceil_div(size_t{kChunk} * G, kItem));
+  unsigned* irun = static_cast<unsigned*>(scratch(sizeof(unsigned) * NK * IS, 14));
   unsigned* totals = small;
   unsigned* nrun = small + (NK + 1);
   double* partial = static_cast<double*>(scratch(sizeof(double) * 2 * M1 * max_items, 12));
@@ -849,12 +855,12 @@ void sync_forward_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, 
   }
   RGN_HIP(hipGetLastError());
   stage_mark(st, "binned_bucket");
-  binned_scan_rows<<<plan.NK, kRowThreads, 0, st>>>(runs, G, cvoff, cidx, nrun, totals);
+  binned_scan_rows<<<plan.NK, kRowThreads, 0, st>>>(runs, G, cvoff, cidx, nrun, totals, irun, IS);
   RGN_HIP(hipGetLastError());
   runs_clean = clean_after;
   stage_mark(st, "binned_scan_rows");
   binned_subhist<<<static_cast<unsigned>(max_items), kSubThreads, subhist_smem(plan), st>>>(
-    P, G, totals, cvoff, cidx, nrun, sortWP, partial);
+    P, G, totals, cvoff, cidx, nrun, irun, IS, sortWP, partial);
   RGN_HIP(hipGetLastError());
   stage_mark(st, "binned_subhist");
   binned_prefix<<<plan.NK, kPrefThreads, sizeof(double) * 2 * M1, st>>>(plan.M, totals, partial, P0, P1);
