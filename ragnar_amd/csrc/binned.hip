@@ -31,12 +31,18 @@
 // switches from segment m to m - 1, and every particle reads one entry.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <limits>
 #include <vector>
 
 #include "emission.hpp"
 #include "rgn_core.hpp"
 
 namespace rgn {
+
+namespace {
+constexpr int kLutCells = 2048;  // sub-cell lookup cells in a sub-histogram block
+}  // namespace
 
 void build_binned_tables(SyncPlan& plan, const std::vector<float>& xs, const std::vector<float>& ys) {
   const int T = plan.T;
@@ -99,18 +105,35 @@ void build_binned_tables(SyncPlan& plan, const std::vector<float>& xs, const std
 
   plan.alpha.reset(std::max<size_t>(1, alphas.size()) * sizeof(double));
   plan.athr.reset(std::max<size_t>(1, alphas.size()) * sizeof(float));
+  std::vector<float> athr(alphas.size());
+  for (size_t i = 0; i < alphas.size(); ++i) {
+    float f = static_cast<float>(alphas[i]);
+    if (static_cast<double>(f) <= alphas[i]) f = std::nextafter(f, 2.0f);
+    athr[i] = f;
+  }
   if (!alphas.empty()) {
     RGN_HIP(hipMemcpy(plan.alpha.ptr, alphas.data(), alphas.size() * sizeof(double), hipMemcpyHostToDevice));
-    std::vector<float> athr(alphas.size());
-    for (size_t i = 0; i < alphas.size(); ++i) {
-      float f = static_cast<float>(alphas[i]);
-      if (static_cast<double>(f) <= alphas[i]) f = std::nextafter(f, 2.0f);
-      athr[i] = f;
-    }
     RGN_HIP(hipMemcpy(plan.athr.ptr, athr.data(), athr.size() * sizeof(float), hipMemcpyHostToDevice));
   }
   plan.jlut.reset(jlut.size() * sizeof(unsigned short));
   RGN_HIP(hipMemcpy(plan.jlut.ptr, jlut.data(), plan.jlut.bytes, hipMemcpyHostToDevice));
+  {
+    // the sub-histogram's lookup cells c of [c / JQ2, (c + 1) / JQ2): the alphas
+    // below the cell, a flag for cells holding two alphas or more, and the
+    // first threshold (as float bits)
+    const int JQ2 = std::min(JQ, kLutCells), r = JQ / JQ2;
+    std::vector<uint2> lut(JQ2);
+    for (int c = 0; c < JQ2; ++c) {
+      const unsigned j0 = jlut[c * r];
+      const unsigned jn = c + 1 < JQ2 ? jlut[(c + 1) * r] : static_cast<unsigned>(plan.M);
+      const float thr = j0 < static_cast<unsigned>(plan.M) ? athr[j0] : std::numeric_limits<float>::infinity();
+      unsigned bits;
+      std::memcpy(&bits, &thr, sizeof(bits));
+      lut[c] = make_uint2(j0 | (jn - j0 >= 2 ? 0x10000u : 0u), bits);
+    }
+    plan.sublut.reset(lut.size() * sizeof(uint2));
+    RGN_HIP(hipMemcpy(plan.sublut.ptr, lut.data(), plan.sublut.bytes, hipMemcpyHostToDevice));
+  }
   plan.rptr.reset(rptr.size() * sizeof(int));
   RGN_HIP(hipMemcpy(plan.rptr.ptr, rptr.data(), plan.rptr.bytes, hipMemcpyHostToDevice));
   plan.rlist.reset(rlist.size() * sizeof(int));
@@ -132,7 +155,6 @@ constexpr unsigned kItem = 16384;          // sources per sub-histogram work ite
 constexpr int kSubThreads = 512;           // threads of a sub-histogram work item
 constexpr int kBatch = 4;                  // sources per lane with loads in flight together
 constexpr int kWin = 2048;                 // runs per LDS window of a work item
-constexpr int kLutCells = 2048;            // sub-cell lookup cells in a sub-histogram block
 constexpr int kPrefThreads = 1024;         // 4 item groups x 256 sub-cells per prefix block
 constexpr int kRowThreads = 1024;          // one row of run descriptors per scan block
 
@@ -140,6 +162,7 @@ struct BinnedDev {
   const double* alpha;
   const float* athr;
   const unsigned short* jlut;
+  const uint2* sublut;
   const int* rptr;
   const int* rlist;
   const BinInfo* binfo;
@@ -151,7 +174,7 @@ struct BinnedDev {
 };
 
 BinnedDev binned_view(const SyncPlan& p) {
-  return BinnedDev{p.alpha.as<double>(), p.athr.as<float>(), p.jlut.as<unsigned short>(), p.rptr.as<int>(), p.rlist.as<int>(),
+  return BinnedDev{p.alpha.as<double>(), p.athr.as<float>(), p.jlut.as<unsigned short>(), p.sublut.as<uint2>(), p.rptr.as<int>(), p.rlist.as<int>(),
                    p.binfo.as<BinInfo>(), p.seg.as<SegInfo>(), p.bine.as<float>(),
                    p.M, p.JQ, p.T, p.kmin, p.NK, p.nbins, emit_const(p), p.K};
 }
@@ -449,15 +472,7 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
     m1[i] = 0.0;
   }
   for (int i = threadIdx.x; i < P.M; i += kSubThreads) athr[i] = P.athr[i];
-  // lookup cell c of [c / JQ2, (c + 1) / JQ2): the alphas below it, the first
-  // threshold, and whether the cell holds two alphas or more
-  for (int c = threadIdx.x; c < JQ2; c += kSubThreads) {
-    const int r = P.JQ / JQ2;
-    const unsigned j0 = P.jlut[c * r];
-    const unsigned jn = c + 1 < JQ2 ? P.jlut[(c + 1) * r] : static_cast<unsigned>(P.M);
-    const float thr = j0 < static_cast<unsigned>(P.M) ? P.athr[j0] : INFINITY;
-    lut[c] = make_uint2(j0 | (jn - j0 >= 2 ? 0x10000u : 0u), __float_as_uint(thr));
-  }
+  for (int c = threadIdx.x; c < JQ2; c += kSubThreads) lut[c] = P.sublut[c];
   __syncthreads();
   const unsigned v0 = (it - first) * kItem;
   const unsigned v1 = min(totals[k], v0 + kItem);

@@ -41,6 +41,7 @@ struct SyncPlan {
   DeviceBuffer athr;               // float[M]: smallest float above alpha[i] (so, for a float
                                    //   phi, alpha[i] < phi exactly when athr[i] <= phi)
   DeviceBuffer jlut;               // uint16[JQ]: number of alphas below q / JQ (JQ >= 8 M)
+  DeviceBuffer sublut;             // uint2[min(JQ, 2048)]: the sub-histogram's lookup cells
   DeviceBuffer rptr;               // int[M + 1]: live bins with alpha rank r are
   DeviceBuffer rlist;              //   rlist[rptr[r] .. rptr[r + 1]) (ascending e)
   DeviceBuffer binfo;              // BinInfo[nbins]
