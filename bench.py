@@ -600,12 +600,15 @@ def main():
             T.sync_adjoint(plan, U, B, w, None, gbar, need=("w", "u", "b"))
 
         ms_fa = event_time(fwd_adj, max(3, args.steps // 2), 3, torch)
+        # the same weighted forward alone: the ratio compares like with like
+        ms_fw = event_time(lambda: T.sync_spectrum(plan, U, B, w), max(3, args.steps // 2), 3, torch)
         gones = torch.ones(nb, device=dev)
         adj_ms = {m: event_time(lambda: T.sync_adjoint(plan, U, B, w, None, gones, need=("w", "u", "b"),
                                                        method=m), 10, 3, torch)
                   for m in ("direct", "binned")}
-        extras["fwd_adj"] = {"ms": ms_fa, "evals_per_s": evals / (ms_fa * 1e-3),
-                             "ratio_to_fwd": ms_fa / ms, "adjoint_ms_by_method": adj_ms}
+        extras["fwd_adj"] = {"ms": ms_fa, "evals_per_s": evals / (ms_fa * 1e-3), "ms_fwd_weighted": ms_fw,
+                             "ratio_to_fwd": ms_fa / ms_fw, "ratio_to_unweighted_fwd": ms_fa / ms,
+                             "adjoint_ms_by_method": adj_ms}
         # differentiable fit at 2^26 particles (fwd + adjoint of the reweighting loss)
         try:
             extras["fit_2p26"] = bench_fit(T, workloads, orc, torch, table, bins)
