@@ -144,6 +144,23 @@ def test_spectrum_edge_cases(orc, table, method):
     assert torch.count_nonzero(T.sync_spectrum(plan, z, z, method=method)) == 0
 
 
+@pytest.mark.parametrize("extra", [0, 100, 8191])
+def test_binned_one_bucket_item_boundaries(orc, table, extra):
+    """Every particle in one bucket: each 8192-particle chunk is one run, so
+    the factorised form's work items (16384 sources) start and end exactly on
+    run starts, with a partial last item or none."""
+    n = 6 * 8192 + extra
+    rng = np.random.default_rng(extra)
+    u = np.zeros((3, n), np.float32)
+    u[0] = 10.0
+    b = np.zeros((3, n), np.float32)
+    b[2] = 1.0
+    w = rng.uniform(0.5, 2.0, n).astype(np.float32)
+    bins = orc.logspace(1e-2, 1e7, 256)
+    S, ex, f64 = _spectrum_case(orc, table, u, b, bins, w=w, method="binned")
+    assert_spectrum(S, ex, f64, what=f"one bucket, n={n}")
+
+
 @pytest.mark.parametrize("method", METHODS)
 def test_spectrum_invariants_large(orc, table, method):
     n = 1 << 21
