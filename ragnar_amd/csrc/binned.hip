@@ -155,7 +155,7 @@ constexpr unsigned kItem = 16384;          // sources per sub-histogram work ite
 constexpr int kSubThreads = 512;           // threads of a sub-histogram work item
 constexpr int kBatch = 4;                  // sources per lane with loads in flight together
 constexpr int kWin = 2048;                 // runs per LDS window of a work item
-constexpr int kPrefThreads = 1024;         // 4 item groups x 256 sub-cells per prefix block
+constexpr int kPrefThreads = 768;          // 3 item groups x 256 sub-cells per prefix block (two blocks per CU)
 constexpr int kRowThreads = 1024;          // one row of run descriptors per scan block
 
 struct BinnedDev {
@@ -337,7 +337,7 @@ __global__ void __launch_bounds__(kBucketThreads) binned_bucket(ParticlesDev D, 
 // One exclusive scan over blocks of (run length, non-empty flag) packed in 64 bits.
 // The descriptors read are zeroed again, so the table needs no clearing pass
 // before the next call.
-__global__ void __launch_bounds__(kRowThreads) binned_scan_rows(unsigned* __restrict__ runs, int G,
+__global__ void __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(8))) binned_scan_rows(unsigned* __restrict__ runs, int G,
                                                              unsigned* __restrict__ cvoff,
                                                              unsigned* __restrict__ cidx,
                                                              unsigned* __restrict__ nrun,
@@ -351,7 +351,8 @@ __global__ void __launch_bounds__(kRowThreads) binned_scan_rows(unsigned* __rest
   const int per = (G + kRowThreads - 1) / kRowThreads;
   const int i0 = min(G, static_cast<int>(threadIdx.x) * per), i1 = min(G, i0 + per);
   // up to kRowCache descriptors per thread stay in registers between the passes
-  constexpr int kRowCache = 8;
+  // (G <= 4096 blocks, 2^25 particles; 64 VGPRs, so two blocks share a CU)
+  constexpr int kRowCache = 4;
   unsigned cache[kRowCache];
   unsigned long long acc = 0;
   if (per <= kRowCache) {
@@ -589,7 +590,7 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
   }
 }
 
-// Pass 4: per bucket, its items summed (four interleaved item groups, combined
+// Pass 4: per bucket, its items summed (three interleaved item groups, combined
 // in a fixed order), then inclusive prefix sums over sub-cells:
 // P0[k][j] = sum_{j' <= j} m0, P1 likewise.
 __global__ void __launch_bounds__(kPrefThreads) binned_prefix(int M, const unsigned* __restrict__ totals,
@@ -597,7 +598,8 @@ __global__ void __launch_bounds__(kPrefThreads) binned_prefix(int M, const unsig
                                                               double* __restrict__ P0, double* __restrict__ P1) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double part[kPrefThreads / 64];
-  __shared__ double red[2][4][256];
+  constexpr int kGroups = kPrefThreads / 256;
+  __shared__ double red[2][kGroups][256];
   const int M1 = M + 1;
   double* s0 = reinterpret_cast<double*>(smem);
   double* s1 = s0 + M1;
@@ -612,14 +614,14 @@ __global__ void __launch_bounds__(kPrefThreads) binned_prefix(int M, const unsig
     double a[4] = {0.0, 0.0, 0.0, 0.0}, b[4] = {0.0, 0.0, 0.0, 0.0};
     if (j < M1) {
       unsigned it = i0 + grp;
-      for (; it + 12 < i1; it += 16) {
+      for (; it + 3 * kGroups < i1; it += 4 * kGroups) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          a[q] += partial[static_cast<size_t>(it + 4 * q) * 2 * M1 + j];
-          b[q] += partial[static_cast<size_t>(it + 4 * q) * 2 * M1 + M1 + j];
+          a[q] += partial[static_cast<size_t>(it + kGroups * q) * 2 * M1 + j];
+          b[q] += partial[static_cast<size_t>(it + kGroups * q) * 2 * M1 + M1 + j];
         }
       }
-      for (int q = 0; it < i1; it += 4, ++q) {  // at most three left
+      for (int q = 0; it < i1; it += kGroups, ++q) {  // at most three left
         a[q] += partial[static_cast<size_t>(it) * 2 * M1 + j];
         b[q] += partial[static_cast<size_t>(it) * 2 * M1 + M1 + j];
       }
@@ -628,8 +630,14 @@ __global__ void __launch_bounds__(kPrefThreads) binned_prefix(int M, const unsig
     red[1][grp][col] = (b[0] + b[1]) + (b[2] + b[3]);
     __syncthreads();
     if (grp == 0 && j < M1) {
-      s0[j] = ((red[0][0][col] + red[0][1][col]) + red[0][2][col]) + red[0][3][col];
-      s1[j] = ((red[1][0][col] + red[1][1][col]) + red[1][2][col]) + red[1][3][col];
+      double t0 = red[0][0][col], t1 = red[1][0][col];
+#pragma unroll
+      for (int g = 1; g < kGroups; ++g) {
+        t0 += red[0][g][col];
+        t1 += red[1][g][col];
+      }
+      s0[j] = t0;
+      s1[j] = t1;
     }
     __syncthreads();
   }
