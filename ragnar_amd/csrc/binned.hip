@@ -147,8 +147,8 @@ void build_binned_tables(SyncPlan& plan, const std::vector<float>& xs, const std
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kBucketThreads = 512;        // threads of a bucket block (two per CU)
-constexpr int kPer = 16;                   // particles per thread in the bucket pass
+constexpr int kBucketThreads = 1024;       // threads of a bucket block (two per CU at 64 VGPRs)
+constexpr int kPer = 8;                    // particles per thread in the bucket pass
 constexpr int kChunk = kBucketThreads * kPer;  // particles per bucket block
 static_assert(kChunk <= 32768, "run descriptors pack start and length in 16 bits each");
 constexpr unsigned kItem = 16384;          // sources per sub-histogram work item
