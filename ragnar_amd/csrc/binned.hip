@@ -152,6 +152,7 @@ constexpr int kPer = 8;                    // particles per thread in the bucket
 constexpr int kChunk = kBucketThreads * kPer;  // particles per bucket block
 static_assert(kChunk <= 32768, "run descriptors pack start and length in 16 bits each");
 constexpr unsigned kItem = 16384;          // sources per sub-histogram work item
+static_assert(kChunk <= kItem, "a run (at most one chunk) must hold at most one work item's first source");
 constexpr int kSubThreads = 512;           // threads of a sub-histogram work item
 constexpr int kBatch = 4;                  // sources per lane with loads in flight together
 constexpr int kWin = 2048;                 // runs per LDS window of a work item
