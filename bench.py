@@ -57,6 +57,9 @@ def parse():
     ap.add_argument("--profile-compute", action="store_true",
                     help="dispatch the compute-bound kernels (pair-by-pair forward, split-FP16 "
                          "contraction) for rocprofv3 VALU / MFMA counters, then exit")
+    ap.add_argument("--summarize-bench", nargs=2, metavar=("TRACE_DIR", "BENCH_JSON"),
+                    help="kernel durations of a rocprofv3 --kernel-trace run of the headline bench "
+                         "command, beside an unprofiled bench line's event-timed stages")
     ap.add_argument("--summarize-compute", nargs=2, metavar=("TRACE_DIR", "PMC_DIR"),
                     help="summarise a --kernel-trace --stats run and a --pmc run of "
                          "--profile-compute into profiles/compute_units.md, then exit")
@@ -306,6 +309,42 @@ def summarize(trace_dir, fetch_dir, write_dir, method):
     print("\n".join(md))
 
 
+def summarize_bench(trace_dir, bench_json):
+    """Durations of every dispatch of the headline step's kernels in a
+    rocprofv3 --kernel-trace run of `bench.py --no-extras --no-cpu`, against
+    the event-timed stages of an unprofiled bench line (bench_json: a file
+    whose last line is that JSON).  Writes profiles/bench_headline_2p24.md."""
+    import csv
+    import statistics
+
+    rows = list(csv.DictReader(open(Path(trace_dir) / "run_kernel_trace.csv")))
+    line = json.loads(Path(bench_json).read_text().strip().splitlines()[-1])
+    stages = line.get("stages_ms", {})
+    out = ["| kernel | dispatches | traced avg (us) | median (us) | min (us) | max (us) | bench line, event-timed (us) |",
+           "|---|---|---|---|---|---|---|"]
+    for k in stages:
+        d = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3 for r in rows
+             if _short_name(r["Kernel_Name"]) == k]
+        if not d:
+            continue
+        out.append(f"| `{k}` | {len(d)} | {statistics.mean(d):.1f} | {statistics.median(d):.1f} | "
+                   f"{min(d):.1f} | {max(d):.1f} | {stages[k] * 1e3:.1f} |")
+    rf = line.get("roofline", {})
+    md = ["# Headline step under rocprofv3: 2^24 particles x 256 bins, default method", "",
+          "Trace: `rocprofv3 --kernel-trace --stats --output-format csv -d <trace> -o run -- "
+          "python3 bench.py --no-extras --no-cpu`;",
+          "summary: `python bench.py --summarize-bench <trace> <bench line>`, the bench line being the",
+          "same command run without the profiler, just before, on the same box.  Every dispatch of the",
+          "step's kernels in the traced run is counted (warm-up, timed steps and stage timing); an",
+          "event-timed stage also holds the gap since the previous launch.", "",
+          *out, "",
+          f"Bench line: {line.get('ms_per_step', 0) * 1e3:.1f} us per step, roofline kernel "
+          f"`{rf.get('kernel')}` at {rf.get('achieved', 0):.0f} {rf.get('unit', '')} "
+          f"(frac {rf.get('frac', 0):.3f}).", ""]
+    (ROOT / "profiles" / "bench_headline_2p24.md").write_text("\n".join(md))
+    print("\n".join(md))
+
+
 def event_time(fn, steps, warmup, torch):
     """Average ms per call of fn, timed with events on the current (launch) stream."""
     for _ in range(warmup):
@@ -504,7 +543,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(multi-GPU sharding is described in DESIGN.md, not launched)"}))
+                                   "(the multi-GPU sharding is designed but not launched)"}))
         sys.exit(2)
 
     if args.summarize:
@@ -512,6 +551,9 @@ def main():
         return
     if args.summarize_compute:
         summarize_compute(*args.summarize_compute)
+        return
+    if args.summarize_bench:
+        summarize_bench(*args.summarize_bench)
         return
     if args.profile:
         profile_run(args)
