@@ -334,7 +334,7 @@ def summarize_bench(trace_dir, bench_json):
           "Trace: `rocprofv3 --kernel-trace --stats --output-format csv -d <trace> -o run -- "
           "python3 bench.py --no-extras --no-cpu`;",
           "summary: `python bench.py --summarize-bench <trace> <bench line>`, the bench line being the",
-          "same command run without the profiler, just before, on the same box.  Every dispatch of the",
+          "same command run without the profiler, just before, on the same machine.  Every dispatch of the",
           "step's kernels in the traced run is counted (warm-up, timed steps and stage timing); an",
           "event-timed stage also holds the gap since the previous launch.", "",
           *out, "",
