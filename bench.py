@@ -118,8 +118,9 @@ PEAK_MFMA_F16 = 2.5e15         # dense
 
 def profile_compute_run(args):
     """Single dispatches of the compute-bound kernels for rocprofv3: the
-    pair-by-pair forward at 2^24 x 256 (FP32 VALU) and the split-FP16 batched
-    contraction at 65536 x 512 x 512, three and one products (MFMA)."""
+    pair-by-pair forward at 2^24 x 256 (FP32 VALU), the split-FP16 batched
+    contraction at 65536 x 512 x 512, three and one products (MFMA), and the
+    inverse-Compton spectrum at 1000 x 1000 x 1000 triples (FP32/FP64 VALU)."""
     import torch
 
     from ragnar_amd import torch_ops as T
@@ -150,6 +151,16 @@ def profile_compute_run(args):
         print(f"profile: contraction {name} x{reps}", flush=True)
         for _ in range(reps):
             T.sync_fromdist_batched(cplan, G, Fm, True, prec, out=out)
+    gi = orc.logspace(1e3, 1e7, 1000)
+    si = orc.logspace(1e-11, 1e-7, 1000)
+    ic = [torch.from_numpy(a).cuda() for a in (gi, orc.generator("plaw", [-1.5, 1e3, 1e7], gi), si,
+                                                 orc.generator("delta", [1e-8, 1e-9], si),
+                                                 orc.logspace(1e3, 1e7, 1000))]
+    T.ic_spectrum(ic[0], ic[1], True, ic[2], ic[3], ic[4])
+    torch.cuda.synchronize()
+    print(f"profile: ic spectrum 1000^3 x{reps}", flush=True)
+    for _ in range(reps):
+        T.ic_spectrum(ic[0], ic[1], True, ic[2], ic[3], ic[4])
     torch.cuda.synchronize()
     print("profile: done", flush=True)
 
@@ -204,7 +215,8 @@ def summarize_compute(trace_dir, pmc_dir):
     for k, d in sorted(dur.items(), key=lambda kv: -statistics.mean(kv[1])):
         full = demangle(k)
         short = _short_name(full)
-        if short not in ("forward_kernel", "combine_kernel", "mfma_contract", "pack_kernel_matrix"):
+        if short not in ("forward_kernel", "combine_kernel", "mfma_contract", "pack_kernel_matrix",
+                         "ic_kernel", "ic_combine"):
             continue
         c = vals.get(k, {})
         t = statistics.mean(d[1:] if len(d) > 1 else d)
@@ -227,8 +239,9 @@ def summarize_compute(trace_dir, pmc_dir):
     md = ["# VALU and MFMA rates of the compute-bound kernels (MI355X)", "",
           "Made by `python bench.py --summarize-compute <trace> <pmc>` from two rocprofv3 runs of",
           "`bench.py --profile-compute` (commands in profiles/README.md): the pair-by-pair",
-          "forward at 2^24 particles x 256 bins and the split-FP16 contraction at 65536 x 512 x 512",
-          "(`mfma_contract<3, true>`: three products; `<1, true>`: one).", "",
+          "forward at 2^24 particles x 256 bins, the split-FP16 contraction at 65536 x 512 x 512",
+          "(`mfma_contract<3, true>`: three products; `<1, true>`: one) and the inverse-Compton",
+          "spectrum at 1000 gamma x 1000 soft-photon x 1000 output bins (`ic_kernel`).", "",
           f"Peaks: FP32 VALU {PEAK_FP32_NONPACKED / 1e12:.1f} TFLOP/s without packed math "
           f"({PEAK_FP32 / 1e12:.1f} with it; this build compiles with packed FP32 off), "
           f"FP64 VALU {PEAK_FP64_VALU / 1e12:.1f}, dense F16 MFMA {PEAK_MFMA_F16 / 1e12:.0f}.",
