@@ -13,7 +13,11 @@
 // Layout: one workgroup per (output bin, chunk of gamma bins), enough of them
 // to fill the device; the threads take the soft-photon bins (the two
 // distributions are a few KB and stay in cache); a fixed-order tree per block
-// and a second pass adding the chunks in order.  No table or plan.
+// and a second pass adding the chunks in order.  No table or plan.  The kernel
+// is bound by its VALU work (IEEE float and double divides), so the quotients
+// that depend on one operand only -- e_ic / g per gamma bin, f_s / e_s per
+// soft-photon bin -- are formed once (the same float values the reference
+// forms per triple) and the triples loop over gamma bins staged in LDS.
 #include <algorithm>
 #include <cmath>
 
@@ -23,6 +27,7 @@ namespace rgn {
 namespace {
 
 constexpr int kIcThreads = 256;
+constexpr int kIcStage = 1024;  // gamma bins staged in LDS at a time
 
 // ic.hpp:20-25.  2 * q * log(q) is float; the 0.5 literal makes the rest double.
 __device__ __forceinline__ float kn_func(float Gamma, float q) {
@@ -35,30 +40,45 @@ __device__ __forceinline__ float kn_func(float Gamma, float q) {
 }
 
 // One workgroup per (output bin, chunk of gamma bins): its threads take the
-// soft-photon bins of each gamma bin of the chunk, in order; the block's FP64
-// partial goes to partial[chunk][bin].
+// soft-photon bins, and each thread runs its bins over the chunk's gamma bins,
+// staged kIcStage at a time; the block's FP64 partial goes to
+// partial[chunk][bin].  Summation order: per thread, stage by stage, soft-photon
+// bin by bin, gamma bin by gamma bin; then a fixed tree over the threads.
 __global__ void __launch_bounds__(kIcThreads) ic_kernel(const float* __restrict__ g, const float* __restrict__ fg,
                                                         int ng, int g_log, const float* __restrict__ es,
                                                         const float* __restrict__ fes, int nes,
                                                         const float* __restrict__ eic, int gper,
                                                         double* __restrict__ partial) {
   __shared__ double red[kIcThreads];
+  __shared__ float s_g[kIcStage], s_f[kIcStage], s_x[kIcStage];
   const int e = blockIdx.x, neic = gridDim.x;
   const float e_ic = eic[e];
   const int g0 = blockIdx.y * gper, g1 = min(ng, g0 + gper);
   double acc = 0.0;
-  for (int gi = g0; gi < g1; ++gi) {
-    const float g_prtls = g[gi], f_prtls = fg[gi];
+  for (int gs = g0; gs < g1; gs += kIcStage) {
+    const int m = min(kIcStage, g1 - gs);
+    __syncthreads();  // the previous stage has been read
+    for (int i = threadIdx.x; i < m; i += kIcThreads) {
+      const float gp = g[gs + i];
+      s_g[i] = gp;
+      s_f[i] = fg[gs + i];
+      s_x[i] = e_ic / gp;
+    }
+    __syncthreads();
     for (int si = threadIdx.x; si < nes; si += kIcThreads) {
       // ic.hpp:62-85, float arithmetic in the reference's order
-      const float e_soft = es[si], f_soft = fes[si];
-      const float Gamma = 4.0f * g_prtls * e_soft;
-      if (e_ic > g_prtls * Gamma / (1.0f + Gamma)) continue;
-      const float q = (e_ic / g_prtls) / (Gamma * (1.0f - (e_ic / g_prtls)));
-      const float kn = kn_func(Gamma, q);
-      const float term = g_log ? f_prtls * (f_soft / e_soft) * e_ic * e_ic * kn / g_prtls
-                               : f_prtls * (f_soft / e_soft) * e_ic * e_ic * kn / (g_prtls * g_prtls);
-      acc += static_cast<double>(term);
+      const float e_soft = es[si];
+      const float fs_es = fes[si] / e_soft;
+      for (int i = 0; i < m; ++i) {
+        const float g_prtls = s_g[i], f_prtls = s_f[i], x = s_x[i];
+        const float Gamma = 4.0f * g_prtls * e_soft;
+        if (e_ic > g_prtls * Gamma / (1.0f + Gamma)) continue;
+        const float q = x / (Gamma * (1.0f - x));
+        const float kn = kn_func(Gamma, q);
+        const float term = g_log ? f_prtls * fs_es * e_ic * e_ic * kn / g_prtls
+                                 : f_prtls * fs_es * e_ic * e_ic * kn / (g_prtls * g_prtls);
+        acc += static_cast<double>(term);
+      }
     }
   }
   red[threadIdx.x] = acc;
