@@ -116,9 +116,12 @@ struct DistSourceFn {
   }
 };
 
+// With out non-null (a one-block grid) the block writes the spectrum itself,
+// out[e] = e_e * sum, exactly what the combine pass would write for one block.
 template <int NB, class SourceFn>
 __global__ void __launch_bounds__(kThreads) forward_kernel(SourceFn fn, size_t n, PlanDev P,
-                                                           double* __restrict__ partials) {
+                                                           double* __restrict__ partials,
+                                                           float* __restrict__ out) {
   extern __shared__ __align__(16) unsigned char smem[];
   float4* tile = reinterpret_cast<float4*>(smem);
   float2* ltab = reinterpret_cast<float2*>(smem + kThreads * sizeof(float4));
@@ -157,7 +160,8 @@ __global__ void __launch_bounds__(kThreads) forward_kernel(SourceFn fn, size_t n
     for (int k = 0; k < NB; ++k) dacc[k] += static_cast<double>(acc[k]);
   }
 
-  // Fixed-order combine of the four waves, then one column of `partials`.
+  // Fixed-order combine of the four waves, then one column of `partials` (or,
+  // for a one-block grid, the output).
   __syncthreads();
   double* red = reinterpret_cast<double*>(smem);
 #pragma unroll
@@ -168,7 +172,14 @@ __global__ void __launch_bounds__(kThreads) forward_kernel(SourceFn fn, size_t n
     const double v = ((red[(0 * NB + k) * 64 + l] + red[(1 * NB + k) * 64 + l]) +
                       red[(2 * NB + k) * 64 + l]) +
                      red[(3 * NB + k) * 64 + l];
-    partials[static_cast<size_t>(e) * gridDim.x + blockIdx.x] = v;
+    if (out) {
+      if (e < P.nbins) {
+        const float ee = P.bine[e];
+        out[e] = (ee == 0.0f) ? 0.0f : static_cast<float>(static_cast<double>(ee) * v);
+      }
+    } else {
+      partials[static_cast<size_t>(e) * gridDim.x + blockIdx.x] = v;
+    }
   }
 }
 
@@ -467,13 +478,13 @@ int forward_grid(size_t n, int blocks_per_cu) {
 
 template <class SourceFn>
 void launch_forward(const SyncPlan& plan, SourceFn fn, size_t n, int grid, double* partials,
-                    hipStream_t st) {
+                    hipStream_t st, float* out = nullptr) {
   const PlanDev P = device_view(plan);
   const size_t smem = forward_smem(plan);
   switch (plan.NB) {
 #define RGN_FWD_CASE(NB_)                                                          \
   case NB_:                                                                        \
-    forward_kernel<NB_, SourceFn><<<grid, kThreads, smem, st>>>(fn, n, P, partials); \
+    forward_kernel<NB_, SourceFn><<<grid, kThreads, smem, st>>>(fn, n, P, partials, out); \
     break;
     RGN_FWD_CASE(1)
     RGN_FWD_CASE(2)
@@ -543,6 +554,10 @@ void sync_forward(const SyncPlan& plan, const ParticlesDev& D, size_t n, float* 
   const int grid = forward_grid(n, blocks_per_cu);
   double* partials =
     static_cast<double*>(scratch(sizeof(double) * plan.nbins_pad * static_cast<size_t>(grid), 0));
+  if (grid == 1) {  // one block: it writes the spectrum, no combine launch
+    launch_forward(plan, ParticleSourceFn{D}, n, grid, partials, st, out);
+    return;
+  }
   launch_forward(plan, ParticleSourceFn{D}, n, grid, partials, st);
   launch_combine(plan, partials, grid, out, st);
 }
@@ -695,6 +710,11 @@ int rgn_sync_fromdist(const rgn_sync_plan* plan, const float* d_gbeta, const flo
     double* partials =
       static_cast<double*>(scratch(sizeof(double) * p.nbins_pad * static_cast<size_t>(grid), 1));
     hipStream_t st = as_stream(stream);
+    if (grid == 1) {  // one block: it writes the spectrum, no combine launch
+      launch_forward(p, DistSourceFn{d_gbeta, d_f, dist_log}, static_cast<size_t>(ng), grid, partials, st,
+                     d_out);
+      return;
+    }
     launch_forward(p, DistSourceFn{d_gbeta, d_f, dist_log}, static_cast<size_t>(ng), grid,
                    partials, st);
     launch_combine(p, partials, grid, d_out, st);
