@@ -511,12 +511,8 @@ void launch_combine(const SyncPlan& plan, const double* partials, int grid, floa
 }
 
 // Automatic choice between the direct and the factorised kernels from a cost
-// model measured on MI355X (2^16 .. 2^24 particles, 32 .. 512 bins):
-//   direct forward  ~ 18 us + n * c(NB), c = 22, 29, 35, 41, 55, 70, 82, 95 ps
-//                     for NB = 1 .. 8 bins per lane (64 NB bins per wave),
-//   direct adjoint  ~ 25 us + n * nbins / 4.6e12 s,
-//   binned forward  ~ 60 us + n * 13.5 ps,   binned adjoint ~ 28 us + n * 14 ps.
-// RAGNAR_SYNC_METHOD (direct | binned) overrides the automatic choice.
+// model measured on MI355X (2^16 .. 2^24 particles, 32 .. 512 bins; constants
+// below).  RAGNAR_SYNC_METHOD (direct | binned) overrides the automatic choice.
 int resolve_method(const SyncPlan& plan, size_t n, int method, bool adjoint) {
   // the factorised kernels address particles with 32-bit byte offsets
   const bool binned_fits = plan.binned_ok && n < kBinnedMaxParticles;
