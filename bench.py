@@ -33,6 +33,26 @@ FLOP_PER_EVAL_ADJ = 8
 HIST_BYTES_PER_PARTICLE = 12
 
 
+def logspace(start, stop, num):
+    """The reference's Logspace grid, from the `ragnar` module of this package."""
+    import numpy as np
+
+    from ragnar_amd import ragnar as rg
+
+    return np.asarray(rg.Logspace(start, stop, num).as_array(), np.float32)
+
+
+def generator(kind, params, e):
+    """A reference generator ("plaw": p, emin, emax; "delta": energy0, denergy)
+    evaluated at the energies e, from the `ragnar` module of this package."""
+    import numpy as np
+
+    from ragnar_amd import ragnar as rg
+
+    gen = {"plaw": rg.PlawGenerator, "delta": rg.DeltaGenerator}[kind](*params)
+    return np.asarray(gen.compute(rg.Bins(np.ascontiguousarray(e, np.float32))).as_array(), np.float32)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,14 +93,13 @@ def profile_run(args):
 
     from ragnar_amd import torch_ops as T
     from ragnar_amd import workloads
-    from oracle import oracle as orc
 
     n, nb = args.particles, args.bins
     u_h, b_h = workloads.particles(n)
     U = torch.from_numpy(u_h).cuda()
     B = torch.from_numpy(b_h).cuda()
-    plan = T.SyncPlan(T.default_table(), orc.logspace(1e-2, 1e7, nb))
-    hplan = T.HistPlan(orc.logspace(1e-2, 1e3, 200), True)
+    plan = T.SyncPlan(T.default_table(), logspace(1e-2, 1e7, nb))
+    hplan = T.HistPlan(logspace(1e-2, 1e3, 200), True)
     src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
     dst = torch.empty_like(src)
     src.fill_(1.0)
@@ -125,19 +144,18 @@ def profile_compute_run(args):
 
     from ragnar_amd import torch_ops as T
     from ragnar_amd import workloads
-    from oracle import oracle as orc
 
     n, nb = args.particles, args.bins
     u_h, b_h = workloads.particles(n)
     U = torch.from_numpy(u_h).cuda()
     B = torch.from_numpy(b_h).cuda()
-    plan = T.SyncPlan(T.default_table(), orc.logspace(1e-2, 1e7, nb))
+    plan = T.SyncPlan(T.default_table(), logspace(1e-2, 1e7, nb))
     S = torch.empty(nb, device="cuda")
     nd, ng, nbc = 65536, 512, 512
     f, _, _ = workloads.distributions(nd, ng)
-    G = torch.from_numpy(orc.logspace(1, 1e6, ng)).cuda()
+    G = torch.from_numpy(logspace(1, 1e6, ng)).cuda()
     Fm = torch.from_numpy(f).cuda()
-    cplan = T.SyncPlan(T.default_table(), orc.logspace(1e-2, 1e7, nbc))
+    cplan = T.SyncPlan(T.default_table(), logspace(1e-2, 1e7, nbc))
     out = torch.empty(nd, nbc, device="cuda")
     reps = args.profile_reps
     for _ in range(args.warmup):
@@ -151,11 +169,11 @@ def profile_compute_run(args):
         print(f"profile: contraction {name} x{reps}", flush=True)
         for _ in range(reps):
             T.sync_fromdist_batched(cplan, G, Fm, True, prec, out=out)
-    gi = orc.logspace(1e3, 1e7, 1000)
-    si = orc.logspace(1e-11, 1e-7, 1000)
-    ic = [torch.from_numpy(a).cuda() for a in (gi, orc.generator("plaw", [-1.5, 1e3, 1e7], gi), si,
-                                                 orc.generator("delta", [1e-8, 1e-9], si),
-                                                 orc.logspace(1e3, 1e7, 1000))]
+    gi = logspace(1e3, 1e7, 1000)
+    si = logspace(1e-11, 1e-7, 1000)
+    ic = [torch.from_numpy(a).cuda() for a in (gi, generator("plaw", [-1.5, 1e3, 1e7], gi), si,
+                                                 generator("delta", [1e-8, 1e-9], si),
+                                                 logspace(1e3, 1e7, 1000))]
     T.ic_spectrum(ic[0], ic[1], True, ic[2], ic[3], ic[4])
     torch.cuda.synchronize()
     print(f"profile: ic spectrum 1000^3 x{reps}", flush=True)
@@ -387,7 +405,7 @@ def read_traffic(kernel):
     return None
 
 
-def bench_fit(T, workloads, orc, torch, table, bins):
+def bench_fit(T, workloads, torch, table, bins):
     """2^26 particles (same generator as the headline step), weights
     W = |u|^(p + 2.5) exp(-|u| / c) at p = -2.2, c = 3e3, loss
     sum_e (log S_e - log S_e^target)^2 over the 256 bins: one step is the
@@ -420,14 +438,14 @@ def bench_fit(T, workloads, orc, torch, table, bins):
             "fwd_adj_evals_per_s": evals / (ms_fa * 1e-3)}
 
 
-def bench_cells(T, workloads, orc, torch, table):
+def bench_cells(T, workloads, torch, table):
     """4096 cells, lognormal counts (median 4000, sigma 1): one persistent launch
     in a hipGraph against 4096 calls of the single-population form."""
     import numpy as np
 
     u, b, offs, _, _ = workloads.cells(4096, 4000, 1.0)
     nb = 256
-    plan = T.SyncPlan(table, orc.logspace(1e-2, 1e7, nb))
+    plan = T.SyncPlan(table, logspace(1e-2, 1e7, nb))
     U = torch.from_numpy(u).cuda()
     B = torch.from_numpy(b).cuda()
     out = torch.empty(4096, nb, device="cuda")
@@ -448,35 +466,29 @@ def bench_cells(T, workloads, orc, torch, table):
             "frac_fp32_peak": 6 * total * nb / (ms_graph * 1e-3) / PEAK_FP32}
 
 
-def bench_contraction(T, workloads, orc, torch, table):
+def bench_contraction(T, workloads, torch, table):
     """65536 power-law x cutoff distributions on 512 log gamma bins -> 512 photon
     bins: FP32 GEMM (rocBLAS) and the split-FP16 MFMA kernel (three and one
-    products); error against the FP64 oracle on 32 rows, over bins above 1e-6
-    of each row's peak."""
-    import numpy as np
-
+    products).  Returns the timings and, for the checker, 32 sampled rows of each
+    output with their inputs."""
     nd, ng, nb = 65536, 512, 512
     f, _, _ = workloads.distributions(nd, ng)
-    g = orc.logspace(1, 1e6, ng)
-    bins = orc.logspace(1e-2, 1e7, nb)
+    g = logspace(1, 1e6, ng)
+    bins = logspace(1e-2, 1e7, nb)
     plan = T.SyncPlan(table, bins)
     G = torch.from_numpy(g).cuda()
     Fm = torch.from_numpy(f).cuda()
     out = torch.empty(nd, nb, device="cuda")
     res = {}
+    rows = list(range(0, nd, nd // 32))
+    samples = {"g": g, "bins": bins, "f": f[rows], "out": {}}
     for name, prec in (("fp32", 0), ("f16x3", 1), ("f16x1", 2)):
         ms = event_time(lambda: T.sync_fromdist_batched(plan, G, Fm, True, prec, out=out), 10, 2, torch)
-        S = out.cpu().numpy()
-        worst = 0.0
-        for d in range(0, nd, nd // 32):
-            _, f64 = orc.sync_fromdist(g, f[d], True, bins, mode=orc.FP64, want_fp64=True)
-            live = f64 > 1e-6 * f64.max()
-            worst = max(worst, float(np.max(np.abs(S[d][live] / f64[live] - 1))))
+        samples["out"][name] = out[rows].cpu().numpy()
         flop = 2.0 * nd * ng * nb
         res[name] = {"ms": ms, "TFLOP_per_s": flop / (ms * 1e-3) / 1e12,
-                     "GB_per_s": (nd * ng + nd * nb) * 4 / (ms * 1e-3) / 1e9,
-                     "max_rel_err_vs_fp64_32rows": worst}
-    return res
+                     "GB_per_s": (nd * ng + nd * nb) * 4 / (ms * 1e-3) / 1e9}
+    return res, samples
 
 
 def stage_profile(T, plan, U, B, out, torch, reps):
@@ -527,6 +539,33 @@ def roofline(prof, n, nb, plan):
     return {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
             "frac": achieved / PEAK_HBM, "traffic": read_traffic(name), "kernel": name,
             "kernel_ms": ms, "algorithmic_bytes": nbytes}
+
+
+def oracle_checks(u, b, bins, S_sub, contraction_samples):
+    """The checker: the timed path's outputs against the oracle (FP64 sums of
+    the reference's float terms).  S_sub is the default-method spectrum of the
+    first len(...) particles; contraction_samples are sampled rows of the batched
+    contraction (or None)."""
+    import numpy as np
+
+    from oracle import oracle as orc
+
+    out = {}
+    _, ref = orc.sync_spectrum(u, b, bins, want_fp64=True)
+    live = ref > 1e-12 * ref.max()
+    out["parity_max_rel_err_subset"] = float(np.max(np.abs(S_sub[live] / ref[live] - 1)))
+    if contraction_samples is not None:
+        g, cb, f = contraction_samples["g"], contraction_samples["bins"], contraction_samples["f"]
+        errs = {}
+        for name, rows in contraction_samples["out"].items():
+            worst = 0.0
+            for i in range(f.shape[0]):
+                _, f64 = orc.sync_fromdist(g, f[i], True, cb, mode=orc.FP64, want_fp64=True)
+                lv = f64 > 1e-6 * f64.max()
+                worst = max(worst, float(np.max(np.abs(rows[i][lv] / f64[lv] - 1))))
+            errs[name] = worst
+        out["contraction_max_rel_err_vs_fp64_32rows"] = errs
+    return out
 
 
 def cpu_baseline(u, b, bins, target_cpu_seconds):
@@ -580,13 +619,12 @@ def main():
 
     from ragnar_amd import torch_ops as T
     from ragnar_amd import workloads
-    from oracle import oracle as orc  # checker and CPU baseline only
 
     torch.cuda.init()
     dev = "cuda"
     n, nb = args.particles, args.bins
     u_h, b_h = workloads.particles(n)
-    bins = orc.logspace(1e-2, 1e7, nb)
+    bins = logspace(1e-2, 1e7, nb)
     U = torch.from_numpy(u_h).to(dev)
     B = torch.from_numpy(b_h).to(dev)
     table = T.default_table()
@@ -608,12 +646,9 @@ def main():
     auto = stage_profile(T, plan, U, B, out, torch, reps=20)
     auto_method = "binned" if auto["kernels"][0][0].startswith("binned") else "direct"
 
-    # parity spot check of the timed output against the oracle on a subset
+    # the default method's output on a subset, for the checker (CPU leg)
     sub = 1 << 14
     S_sub = T.sync_spectrum(plan, U[:, :sub].contiguous(), B[:, :sub].contiguous()).cpu().numpy()
-    _, ref = orc.sync_spectrum(u_h[:, :sub], b_h[:, :sub], bins, want_fp64=True)
-    live = ref > 1e-12 * ref.max()
-    parity = float(np.max(np.abs(S_sub[live] / ref[live] - 1)))
     S_full = {m: T.sync_spectrum(plan, U, B, method=m).double().cpu().numpy() for m in ("direct", "binned")}
     live = S_full["direct"] > 1e-12 * S_full["direct"].max()
     methods_agree = float(np.max(np.abs(S_full["binned"][live] / S_full["direct"][live] - 1)))
@@ -639,7 +674,6 @@ def main():
         "methods_ms": method_ms,
         "methods_max_rel_diff": methods_agree,
         "stages_ms": {name: t for name, t in auto["kernels"]},
-        "parity_max_rel_err_subset": parity,
     }
 
     extras = {}
@@ -666,40 +700,49 @@ def main():
                              "adjoint_ms_by_method": adj_ms}
         # differentiable fit at 2^26 particles (fwd + adjoint of the reweighting loss)
         try:
-            extras["fit_2p26"] = bench_fit(T, workloads, orc, torch, table, bins)
+            extras["fit_2p26"] = bench_fit(T, workloads, torch, table, bins)
         except Exception as exc:  # reported, the headline line still prints
             extras["fit_2p26"] = {"error": repr(exc)}
         # energy histogram (HBM bound: 12 B per particle)
-        hbins = orc.logspace(1e-2, 1e3, 200)
+        hbins = logspace(1e-2, 1e3, 200)
         hplan = T.HistPlan(hbins, True)
         ms_h = event_time(lambda: T.energy_hist(hplan, U, True), args.steps, 2, torch)
         extras["energy_hist"] = {"ms": ms_h, "GB_per_s": HIST_BYTES_PER_PARTICLE * n / (ms_h * 1e-3) / 1e9,
                                  "frac_hbm_peak": HIST_BYTES_PER_PARTICLE * n / (ms_h * 1e-3) / PEAK_HBM}
         # distribution form, 200 x 200 (launch-latency bound)
-        g = orc.logspace(1, 100, 200)
-        f = orc.generator("plaw", [-2.0, 1.0, 100.0], g)
-        dplan = T.SyncPlan(table, orc.logspace(0.01, 1e7, 200))
+        g = logspace(1, 100, 200)
+        f = generator("plaw", [-2.0, 1.0, 100.0], g)
+        dplan = T.SyncPlan(table, logspace(0.01, 1e7, 200))
         G, Fd = torch.from_numpy(g).to(dev), torch.from_numpy(f).to(dev)
         ms_d = event_time(lambda: T.sync_fromdist(dplan, G, Fd, True), 50, 5, torch)
         extras["fromdist_200x200"] = {"us": ms_d * 1e3}
         # inverse-Compton spectrum, the reference test's 200 x 200 x 200 triples
-        gi = orc.logspace(1e3, 1e7, 200)
-        si = orc.logspace(1e-11, 1e-7, 200)
-        ti = [torch.from_numpy(a).to(dev) for a in (gi, orc.generator("plaw", [-1.5, 1e3, 1e7], gi), si,
-                                                     orc.generator("delta", [1e-8, 1e-9], si),
-                                                     orc.logspace(1e3, 1e7, 200))]
+        gi = logspace(1e3, 1e7, 200)
+        si = logspace(1e-11, 1e-7, 200)
+        ti = [torch.from_numpy(a).to(dev) for a in (gi, generator("plaw", [-1.5, 1e3, 1e7], gi), si,
+                                                     generator("delta", [1e-8, 1e-9], si),
+                                                     logspace(1e3, 1e7, 200))]
         ms_i = event_time(lambda: T.ic_spectrum(ti[0], ti[1], True, ti[2], ti[3], ti[4]), 50, 5, torch)
         extras["ic_200x200x200"] = {"us": ms_i * 1e3, "triples_per_s": 200 ** 3 / (ms_i * 1e-3)}
-        for name, fn in (("cells_4096", lambda: bench_cells(T, workloads, orc, torch, table)),
+        for name, fn in (("cells_4096", lambda: bench_cells(T, workloads, torch, table)),
                          ("contraction_65536x512x512",
-                          lambda: bench_contraction(T, workloads, orc, torch, table))):
+                          lambda: bench_contraction(T, workloads, torch, table))):
             try:
                 extras[name] = fn()
             except Exception as exc:  # reported, the headline line still prints
                 extras[name] = {"error": repr(exc)}
+    samples = None
+    c = extras.get("contraction_65536x512x512")
+    if isinstance(c, tuple):
+        extras["contraction_65536x512x512"], samples = c
     res["extras"] = extras
 
+    # CPU leg: the oracle times the reference algorithm and checks the outputs
     if not args.no_cpu:
+        checks = oracle_checks(u_h[:, :sub], b_h[:, :sub], bins, S_sub, samples)
+        res["parity_max_rel_err_subset"] = checks["parity_max_rel_err_subset"]
+        for name, err in checks.get("contraction_max_rel_err_vs_fp64_32rows", {}).items():
+            extras["contraction_65536x512x512"][name]["max_rel_err_vs_fp64_32rows"] = err
         res["cpu_baseline"] = cpu_baseline(u_h, b_h, bins, args.cpu_seconds)
     print(json.dumps(res))
 
