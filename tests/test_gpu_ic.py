@@ -40,6 +40,12 @@ def _cases(orc):
     es3 = orc.logspace(1e-9, 1e-3, 64)
     yield ("broad power laws", g3, orc.generator("bplaw", (1e3, -1.5, -3.2, 1.0, 1e6), g3), True, es3,
            orc.generator("plaw", (-0.5, 1e-9, 1e-3), es3), orc.logspace(1e-6, 1e6, 300))
+    # more output bins than the device has workgroup slots per chunk: one chunk
+    # of all 1500 gamma bins, staged through LDS in two passes
+    g4 = orc.logspace(10.0, 1e6, 1500)
+    es4 = orc.logspace(1e-8, 1e-4, 8)
+    yield ("one chunk, two LDS stages", g4, orc.generator("plaw", (-2.0, 10.0, 1e6), g4), True, es4,
+           orc.generator("plaw", (-1.0, 1e-8, 1e-4), es4), orc.logspace(1e-4, 1e6, 2100))
 
 
 def test_ic_vs_oracle(orc):
