@@ -199,15 +199,6 @@ __device__ __forceinline__ int subcell(const double* __restrict__ alpha, const u
   return j;
 }
 
-// The same count for a float phi, with float thresholds: alpha[i] < phi
-// exactly when athr[i] <= phi.
-__device__ __forceinline__ int subcell_f(const float* __restrict__ athr, const unsigned short* __restrict__ jlut,
-                                         int M, int JQ, float phi) {
-  int j = jlut[min(static_cast<int>(phi * JQ), JQ - 1)];
-  while (j < M && athr[j] <= phi) ++j;
-  return j;
-}
-
 // Exclusive prefix of per-thread values over an NT-thread block: a shuffle
 // scan inside each wave, then the wave totals (wsum: NT / 64 words of LDS),
 // added in a fixed order by every thread.  Sets total; every thread calls it.
