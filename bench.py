@@ -18,7 +18,6 @@ from __future__ import annotations
 
 import argparse
 import json
-import os
 import sys
 import time
 from pathlib import Path
@@ -88,7 +87,6 @@ def parse():
 
 def profile_run(args):
     """Single dispatches for rocprofv3 --pmc / --kernel-trace runs."""
-    import numpy as np
     import torch
 
     from ragnar_amd import torch_ops as T
@@ -441,8 +439,6 @@ def bench_fit(T, workloads, torch, table, bins):
 def bench_cells(T, workloads, torch, table):
     """4096 cells, lognormal counts (median 4000, sigma 1): one persistent launch
     in a hipGraph against 4096 calls of the single-population form."""
-    import numpy as np
-
     u, b, offs, _, _ = workloads.cells(4096, 4000, 1.0)
     nb = 256
     plan = T.SyncPlan(table, logspace(1e-2, 1e7, nb))
@@ -571,7 +567,6 @@ def oracle_checks(u, b, bins, S_sub, contraction_samples):
 def cpu_baseline(u, b, bins, target_cpu_seconds):
     """The oracle's faithful restatement (per-thread float accumulators, FP64
     per-pair OmegaSync, log10f interpolation), on a bounded sample."""
-    import numpy as np
     from oracle import oracle as orc
 
     threads = orc.num_threads()
