@@ -109,20 +109,6 @@ inline real_t interp_log(real_t x0, const real_t* x, const real_t* y, std::size_
          std::log10(x[xi + 1] / x[xi]);
 }
 
-// tabulation.hpp:43-52 (LG = false branch)
-inline real_t interp_lin(real_t x0, const real_t* x, const real_t* y, std::size_t n,
-                         real_t xmin, real_t xmax, real_t yfill) {
-  if (x0 < xmin or x0 >= xmax) {
-    return yfill;
-  }
-  const auto xi = static_cast<std::size_t>(static_cast<real_t>(n - 1) *
-                                           std::fabs(x0 - xmin) / (xmax - xmin));
-  if (xi >= n - 1) {
-    return y[n - 1];
-  }
-  return (y[xi + 1] * (x0 - x[xi]) + y[xi] * (x[xi + 1] - x0)) / (x[xi + 1] - x[xi]);
-}
-
 // FP64 twin of interp_log used by ORC_FP64: same float nodes, double arithmetic.
 inline double interp_log_d(double x0, const real_t* x, const real_t* y, std::size_t n,
                            double xmin, double xmax) {
@@ -233,16 +219,17 @@ void orc_tabulate_ffunc(std::size_t n, float xmin, float xmax, float* xs, float*
   }
 }
 
-// tabulation.cpp:84-102 (MinMax reduce) + tabulation.hpp:19-53
-float orc_interp(int log_grid, float x0, const float* x, const float* y, std::size_t n,
-                 float yfill) {
+// tabulation.cpp:84-102 (MinMax reduce) + tabulation.hpp:19-53.  Only the
+// log-grid branch is restated: the reference instantiates its interpolant with
+// LOGGRID = true only (global.h:32; its callers are synchrotron.hpp:82,163), so the linear branch
+// of tabulation.hpp:43-52 is never evaluated.
+float orc_interp(float x0, const float* x, const float* y, std::size_t n, float yfill) {
   float xmin = x[0], xmax = x[0];
   for (std::size_t i = 0; i < n; ++i) {
     xmin = x[i] < xmin ? x[i] : xmin;
     xmax = x[i] > xmax ? x[i] : xmax;
   }
-  return log_grid ? interp_log(x0, x, y, n, xmin, xmax, yfill)
-                  : interp_lin(x0, x, y, n, xmin, xmax, yfill);
+  return interp_log(x0, x, y, n, xmin, xmax, yfill);
 }
 
 // synchrotron.hpp:181-232, exposed for closed-form tests.
@@ -335,6 +322,104 @@ int orc_sync_spectrum(int mode, const float* ux, const float* uy, const float* u
   for (std::size_t e = 0; e < nbins; ++e) {
     out[e] = static_cast<real_t>(sum[e]);
     if (out_d) out_d[e] = sum[e];
+  }
+  return 0;
+}
+
+// Gradient of L = sum_e gbar[e] S[e] for the ORC_FP64 spectrum model (same
+// float table and nodes, double arithmetic), with respect to each particle's
+// weight, momentum, magnetic and electric field.  No reference counterpart: the
+// reference is forward-only.  Exact derivative of that model: the interpolant
+// is piecewise linear in ln x, so dF/dln x is the slope of the segment that
+// interp_log_d evaluates (zero outside the table and past its last node).
+// Per particle, with x_e = e_e / eps, eps = e_at gamma^2 chi / g_syn^2:
+//   A = sum_e gbar_e e_e F(x_e),  C = sum_e gbar_e e_e dF/dln x(x_e),
+//   dL/dw = chi A,  dL/dchi = w (A - C),  dL/dgamma = -2 w chi C / gamma,
+// chained through chi = sqrt(q) / B0, q = |E + beta x B|^2 - (beta.E)^2,
+// beta = u / gamma, gamma = sqrt(1 + u^2):
+//   dq/dE = 2 V - 2 (beta.E) beta,  dq/dB = 2 V x beta,
+//   dq/dbeta = 2 B x V - 2 (beta.E) E,  dbeta/du = (I - beta beta^T) / gamma,
+//   dgamma/du = beta.
+// g* outputs are (3, n) component-major; a null ex/ey/ez or w reads as zero /
+// one.  dnode[p] = the smallest distance, in units of the segment, of any of
+// the particle's x_e from a table node (where dF/dln x jumps).
+int orc_sync_adjoint_fp64(const float* ux, const float* uy, const float* uz, const float* ex,
+                          const float* ey, const float* ez, const float* bx, const float* by,
+                          const float* bz, const float* w, std::size_t np, const float* bins,
+                          std::size_t nbins, const float* fx, const float* fy, std::size_t fn,
+                          double B0, double g_syn, double e_at, const double* gbar, double* gw,
+                          double* gu, double* gb, double* ge, double* dnode) {
+  real_t fxmin = fx[0], fxmax = fx[0];
+  for (std::size_t i = 0; i < fn; ++i) {
+    fxmin = fx[i] < fxmin ? fx[i] : fxmin;
+    fxmax = fx[i] > fxmax ? fx[i] : fxmax;
+  }
+  const double lxmin = std::log10(static_cast<double>(fxmin));
+  const double lspan = std::log10(static_cast<double>(fxmax) / fxmin);
+#pragma omp parallel for schedule(static)
+  for (std::size_t p = 0; p < np; ++p) {
+    const double u[3] = {ux[p], uy[p], uz[p]};
+    const double E[3] = {comp(ex, p), comp(ey, p), comp(ez, p)};
+    const double B[3] = {bx[p], by[p], bz[p]};
+    const double wp = w ? static_cast<double>(w[p]) : 1.0;
+    const double gamma = std::sqrt(1.0 + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    const double be[3] = {u[0] / gamma, u[1] / gamma, u[2] / gamma};
+    const double bdote = be[0] * E[0] + be[1] * E[1] + be[2] * E[2];
+    const double V[3] = {E[0] + (be[1] * B[2] - be[2] * B[1]), E[1] + (be[2] * B[0] - be[0] * B[2]),
+                         E[2] + (be[0] * B[1] - be[1] * B[0])};
+    const double q = V[0] * V[0] + V[1] * V[1] + V[2] * V[2] - bdote * bdote;
+    const double chi = std::sqrt(q) / B0;
+    const double eps = e_at * gamma * gamma * chi / (g_syn * g_syn);
+    double A = 0.0, Cs = 0.0, dmin = 1.0;
+    if (eps > 0.0) {
+      for (std::size_t e = 0; e < nbins; ++e) {
+        const double x0 = bins[e] / eps;
+        if (x0 < fxmin or x0 >= fxmax) continue;
+        auto i = static_cast<std::size_t>(static_cast<double>(fn - 1) *
+                                          std::fabs(std::log10(x0) - lxmin) / lspan);
+        if (i >= fn - 1) {  // the constant y[n-1] (interp_log_d): no slope
+          A += gbar[e] * bins[e] * fy[fn - 1];
+          continue;
+        }
+        // the double index can fall one node off near a node; take the
+        // segment that holds x0
+        if (i + 1 < fn - 1 && x0 >= fx[i + 1]) ++i;
+        if (i > 0 && x0 < fx[i]) --i;
+        const double l = std::log(static_cast<double>(fx[i + 1]) / fx[i]);
+        const double f = std::log(x0 / fx[i]) / l;
+        const double slope = (static_cast<double>(fy[i + 1]) - fy[i]) / l;
+        const double F = fy[i] + f * (static_cast<double>(fy[i + 1]) - fy[i]);
+        A += gbar[e] * bins[e] * F;
+        Cs += gbar[e] * bins[e] * slope;
+        dmin = std::fmin(dmin, std::fmin(f, 1.0 - f));
+      }
+    }
+    dnode[p] = dmin;
+    double o_w = 0.0, o_u[3] = {0, 0, 0}, o_b[3] = {0, 0, 0}, o_e[3] = {0, 0, 0};
+    if (eps > 0.0) {
+      o_w = chi * A;
+      const double gchi = wp * (A - Cs);
+      const double ggam = -2.0 * wp * chi * Cs / gamma;
+      const double dchi_dq = 1.0 / (2.0 * B0 * B0 * chi);
+      const double k = gchi * dchi_dq;
+      const double dqdb[3] = {2 * (B[1] * V[2] - B[2] * V[1]) - 2 * bdote * E[0],
+                              2 * (B[2] * V[0] - B[0] * V[2]) - 2 * bdote * E[1],
+                              2 * (B[0] * V[1] - B[1] * V[0]) - 2 * bdote * E[2]};
+      const double bq = be[0] * dqdb[0] + be[1] * dqdb[1] + be[2] * dqdb[2];
+      for (int c = 0; c < 3; ++c) {
+        o_u[c] = k * (dqdb[c] - be[c] * bq) / gamma + ggam * be[c];
+        o_e[c] = k * (2 * V[c] - 2 * bdote * be[c]);
+      }
+      o_b[0] = k * 2 * (V[1] * be[2] - V[2] * be[1]);
+      o_b[1] = k * 2 * (V[2] * be[0] - V[0] * be[2]);
+      o_b[2] = k * 2 * (V[0] * be[1] - V[1] * be[0]);
+    }
+    gw[p] = o_w;
+    for (int c = 0; c < 3; ++c) {
+      gu[c * np + p] = o_u[c];
+      gb[c * np + p] = o_b[c];
+      ge[c * np + p] = o_e[c];
+    }
   }
   return 0;
 }

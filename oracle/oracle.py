@@ -44,7 +44,7 @@ def lib():
         L.orc_ffunc_integrand.argtypes = [C.c_float]
         L.orc_tabulate_ffunc.argtypes = [C.c_size_t, C.c_float, C.c_float, _f32, _f32]
         L.orc_interp.restype = C.c_float
-        L.orc_interp.argtypes = [C.c_int, C.c_float, _f32, _f32, C.c_size_t, C.c_float]
+        L.orc_interp.argtypes = [C.c_float, _f32, _f32, C.c_size_t, C.c_float]
         L.orc_omega_sync.argtypes = [_f32, C.c_float, C.c_float, C.c_float,
                                      C.POINTER(C.c_float), C.POINTER(C.c_float)]
         L.orc_omega_sync_fp64.argtypes = [_f64, C.c_double, C.c_double, C.c_double,
@@ -56,6 +56,9 @@ def lib():
         L.orc_sync_fromdist.argtypes = [C.c_int, _f32, _f32, C.c_size_t, C.c_int, _f32,
                                         C.c_size_t, _f32, _f32, C.c_size_t, C.c_float,
                                         C.c_float, _f32, vp]
+        L.orc_sync_adjoint_fp64.argtypes = ([vp] * 10 + [C.c_size_t, _f32, C.c_size_t, _f32, _f32,
+                                              C.c_size_t, C.c_double, C.c_double, C.c_double,
+                                              _f64, _f64, _f64, _f64, _f64, _f64])
         L.orc_ic_spectrum.argtypes = [C.c_int, _f32, _f32, C.c_size_t, C.c_int, _f32, _f32,
                                       C.c_size_t, _f32, C.c_size_t, _f32, vp]
         L.orc_energy_hist.argtypes = [C.c_int, _f32, _f32, _f32, C.c_size_t, _f32, C.c_size_t,
@@ -99,8 +102,9 @@ def ftable(n=200, xmin=1e-6, xmax=100.0):
     return x, y
 
 
-def interp(x0, x, y, log_grid=True, yfill=0.0) -> np.float32:
-    return np.float32(lib().orc_interp(int(log_grid), np.float32(x0), np.ascontiguousarray(x, np.float32),
+def interp(x0, x, y, yfill=0.0) -> np.float32:
+    """InterpolateTabulatedFunction<true> on log-spaced nodes (tabulation.hpp:19-42)."""
+    return np.float32(lib().orc_interp(np.float32(x0), np.ascontiguousarray(x, np.float32),
                                        np.ascontiguousarray(y, np.float32), len(x), np.float32(yfill)))
 
 
@@ -149,6 +153,25 @@ def sync_spectrum(u, b, bins, e=None, w=None, B0=1.0, g_syn=1.0, e_at=1.0, mode=
                             *[_ptr(c) for c in bb], _ptr(ww), n, bins, len(bins), fx, fy, len(fx),
                             B0, g_syn, e_at, out, outd.ctypes.data)
     return (out, outd) if want_fp64 else out
+
+
+def sync_adjoint_fp64(u, b, bins, gbar, e=None, w=None, B0=1.0, g_syn=1.0, e_at=1.0, table=None):
+    """FP64 gradient of L = sum_e gbar[e] S[e] (the ORC_FP64 model) per particle:
+    (gw [N], gu [3, N], gb [3, N], ge [3, N], dnode [N]); dnode is the smallest
+    distance of any of the particle's x_e from a table node, in segments."""
+    u = np.ascontiguousarray(u, np.float32)
+    n = u.shape[1]
+    uu, ee, bb = _soa(u, n), _soa(e, n), _soa(b, n)
+    ww = None if w is None else np.ascontiguousarray(w, np.float32)
+    bins = np.ascontiguousarray(bins, np.float32)
+    fx, fy = table if table is not None else ftable()
+    gw, dn = np.zeros(n), np.zeros(n)
+    gu, gb, ge = np.zeros((3, n)), np.zeros((3, n)), np.zeros((3, n))
+    lib().orc_sync_adjoint_fp64(*[_ptr(c) for c in uu], *[_ptr(c) for c in ee], *[_ptr(c) for c in bb],
+                                _ptr(ww), n, bins, len(bins), np.ascontiguousarray(fx, np.float32),
+                                np.ascontiguousarray(fy, np.float32), len(fx), B0, g_syn, e_at,
+                                np.ascontiguousarray(gbar, np.float64), gw, gu, gb, ge, dn)
+    return gw, gu, gb, ge, dn
 
 
 def sync_fromdist(gbeta, f, dist_log, bins, g_syn=1.0, e_at=1.0, mode=EXACT, table=None,

@@ -13,7 +13,11 @@ plausible mistake (a dropped term, a wrong index, a transposed operand) breaks:
  - interpolation (tabulation.hpp:19-53): exact at nodes, continuity, zero
    outside, float64 brute force;
  - OmegaSync_ChiR (synchrotron.hpp:181-232): closed forms for u _|_ B, u || B,
-   u = 0, and an FP64 restatement with E != 0;
+   u = 0; with E != 0, closed forms (E || u, E _|_ u, crossed fields, the
+   force-free drift), the covariant form |F^{mu nu} u_nu| of the Lorentz-force
+   four-vector, and the invariance of gamma chiR under Lorentz boosts;
+ - the FP64 adjoint (no reference counterpart): complex-step derivatives of
+   each particle's loss term;
  - spectrum (synchrotron.hpp:145-172): single-particle closed form, additivity,
    permutation, linearity in w, mode agreement, and the reference's own
    particles-vs-distribution test (src/tests/synchrotron.py:62-104);
@@ -29,6 +33,8 @@ import math
 import numpy as np
 import pytest
 import scipy.special as ss
+
+from _adjoint import loss_per_particle_complex
 
 
 def test_logspace_linspace_vs_numpy(orc):
@@ -80,7 +86,6 @@ def test_ftable_recipe_independent(orc):
     for i in range(25, 182):
         ref = _recipe_numpy(x[i])
         assert abs(ref / y[i] - 1) < 2e-6, (i, x[i], y[i], ref)
-    assert orc.ffunc_integrand(0.5) == y[np.argmin(np.abs(x - 0.5))] or True
     assert abs(orc.ffunc_integrand(0.5) / _recipe_numpy(0.5) - 1) < 2e-6
 
 
@@ -154,21 +159,114 @@ def test_omega_sync_closed_forms(orc):
     assert orc.omega_sync((0, 0, 5.0), (0, 0, 0), (0, 0, 2.0)) == (0.0, 0.0)
     # u = 0, E = 0 -> 0
     assert orc.omega_sync((0, 0, 0), (0, 0, 0), (0.3, 0.1, 1.0)) == (0.0, 0.0)
-    # general case with E != 0 against an FP64 restatement
-    rng = np.random.default_rng(7)
-    for _ in range(50):
-        u, e, b = rng.normal(0, 5, 3), rng.normal(0, 0.2, 3), rng.normal(0, 1, 3)
-        ep, chi = orc.omega_sync_fp64(u, e, b, 1.3, 1.0, 1.0)
-        beta = u / np.sqrt(1 + u @ u)
-        v = e + np.cross(beta, b)
-        q = v @ v - (beta @ e) ** 2
-        assert abs(chi - np.sqrt(q) / 1.3) <= 1e-12 * max(chi, 1)
-        epf, chif = orc.omega_sync(u, e, b, 1.3)
-        assert abs(chif / chi - 1) < 1e-6
     # B -> cB, B0 -> cB0 leaves chi unchanged
     ep1, c1 = orc.omega_sync((2.0, 1.0, 0.5), (0, 0, 0), (0.1, 0.2, 1.0), 1.0)
     ep2, c2 = orc.omega_sync((2.0, 1.0, 0.5), (0, 0, 0), (0.4, 0.8, 4.0), 4.0)
     assert abs(c1 / c2 - 1) < 1e-7
+
+
+def _field_tensor(e, b):
+    """F^{mu nu} in the (+,-,-,-) metric: F^{0i} = -E_i, F^{ij} = -eps_ijk B_k."""
+    F = np.zeros((4, 4))
+    F[0, 1:], F[1:, 0] = -np.asarray(e, float), np.asarray(e, float)
+    F[1, 2], F[2, 1] = -b[2], b[2]
+    F[1, 3], F[3, 1] = b[1], -b[1]
+    F[2, 3], F[3, 2] = -b[0], b[0]
+    return F
+
+
+ETA = np.diag([1.0, -1.0, -1.0, -1.0])
+
+
+def _chi_covariant(u, e, b, B0):
+    """chiR from the Lorentz force four-vector a = F^{mu nu} U_nu, U = (gamma, u):
+    a.a = -gamma^2 ((E + beta x B)^2 - (beta.E)^2), so chiR = sqrt(-a.a) / (gamma B0).
+    Tensor algebra only: no cross product or dot product of the 3-vector form."""
+    u = np.asarray(u, float)
+    g = np.sqrt(1.0 + u @ u)
+    a = _field_tensor(e, b) @ ETA @ np.array([g, *u])
+    return np.sqrt(max(-(a @ ETA @ a), 0.0)) / (g * B0), g
+
+
+def _boost(u, e, b, v):
+    """(u', E', B') in the frame moving with velocity v (Jackson 11.19, 11.149)."""
+    u, e, b, v = (np.asarray(a, float) for a in (u, e, b, v))
+    gv = 1.0 / np.sqrt(1.0 - v @ v)
+    g = np.sqrt(1.0 + u @ u)
+    k = gv * gv / (gv + 1.0)
+    up = u + ((gv - 1.0) * (v @ u) / (v @ v) - gv * g) * v
+    ep = gv * (e + np.cross(v, b)) - k * v * (v @ e)
+    bp = gv * (b - np.cross(v, e)) - k * v * (v @ b)
+    return up, ep, bp
+
+
+def test_omega_sync_electric_field_closed_forms(orc):
+    """E != 0 terms of OmegaSync_ChiR (synchrotron.hpp:190-231): chiR B0 =
+    sqrt((E + beta x B)^2 - (beta.E)^2) in closed forms where one term survives.
+    The float form is held to 1e-6 relative (float inputs, float squares, float
+    chiR); the FP64 twin to 1e-13."""
+    def both(u, e, b, B0=1.0):
+        _, chi64 = orc.omega_sync_fp64(u, e, b, B0)
+        _, chi32 = orc.omega_sync(u, e, b, B0)
+        return chi64, float(chi32)
+
+    f32 = lambda v: float(np.float32(v))  # noqa: E731  (inputs exactly representable in float)
+    for umag, emag, B0 in ((0.3, 0.2, 1.0), (40.0, 1.5, 0.7), (2e3, 3e-2, 2.5)):
+        umag, em = f32(umag), f32(emag)
+        g = np.sqrt(1.0 + umag ** 2)
+        beta = umag / g
+        # E parallel to u, B = 0 (also B parallel to u): chiR = |E| / (gamma B0)
+        for b in ((0, 0, 0), (0.0, 0.0, 0.8)):
+            chi64, chi32 = both((0, 0, umag), (0, 0, -em), b, B0)
+            ref = em / (g * B0)
+            # E^2 - (beta.E)^2 = E^2 / gamma^2 cancels: double rounding grows as gamma^2
+            assert abs(chi64 / ref - 1) < 1e-13 + 1e-15 * g * g and abs(chi32 / ref - 1) < 1e-6
+        # E perpendicular to u, B = 0: chiR = |E| / B0
+        chi64, chi32 = both((umag, 0, 0), (0, em, 0), (0, 0, 0), B0)
+        assert abs(chi64 / (em / B0) - 1) < 1e-13 and abs(chi32 / (em / B0) - 1) < 1e-6
+        # crossed fields, u along x, E along y, B along z: beta x B = (0, -beta Bz, 0),
+        # so chiR = |Ey - beta Bz| / B0; Ey = -beta Bz doubles the magnetic term
+        bz = f32(1.3)
+        for ey in (-beta * bz, 0.5 * beta * bz, 3.0 * beta * bz):
+            ey = f32(ey)
+            ref = abs(ey - beta * bz) / B0
+            chi64, chi32 = both((umag, 0, 0), (0, ey, 0), (0, 0, bz), B0)
+            assert abs(chi64 / ref - 1) < 1e-12 and abs(chi32 / ref - 1) < 1e-6
+        # force-free drift E = -beta x B (beta perpendicular to B): no emission;
+        # the float E leaves a residue of a few float ulps of |beta B|
+        u = np.array([umag * 0.6, umag * 0.8, 0.0])
+        gg = np.sqrt(1.0 + u @ u)
+        bb = np.array([0.0, 0.0, 1.7])
+        e = -np.cross(u / gg, bb)
+        chi64, _ = orc.omega_sync_fp64(u, e, bb, B0)
+        assert chi64 <= 1e-14 * np.linalg.norm(u / gg) * 1.7
+        e32 = e.astype(np.float32)
+        ep, chi32 = orc.omega_sync(u.astype(np.float32), e32, bb.astype(np.float32), B0)
+        assert float(chi32) <= 1e-6 * np.linalg.norm(u / gg) * 1.7 / B0
+
+
+def test_omega_sync_covariant_and_boost_invariant(orc):
+    """General E, B, u: the oracle's chiR (FP64 twin to 1e-12, float form to
+    1e-6) equals the covariant form sqrt(-a.a)/(gamma B0) of the Lorentz-force
+    four-vector, and gamma chiR is the same in a boosted frame."""
+    rng = np.random.default_rng(7)
+    for _ in range(200):
+        u = rng.normal(0, 5, 3) * 10 ** rng.uniform(-2, 2)
+        e, b = rng.normal(0, 0.4, 3), rng.normal(0, 1, 3)
+        B0 = 10 ** rng.uniform(-1, 1)
+        chi_cov, g = _chi_covariant(u, e, b, B0)
+        _, chi64 = orc.omega_sync_fp64(u, e, b, B0)
+        assert abs(chi64 - chi_cov) <= 1e-12 * max(chi_cov, 1e-300) + 1e-15 * np.linalg.norm(b)
+        uf, ef, bf = (a.astype(np.float32) for a in (u, e, b))
+        chi_cov32, _ = _chi_covariant(uf, ef, bf, B0)
+        _, chi32 = orc.omega_sync(uf, ef, bf, B0)
+        assert abs(float(chi32) / chi_cov32 - 1) < 1e-6
+        v = rng.normal(0, 1, 3)
+        v *= rng.uniform(0.1, 0.9) / np.linalg.norm(v)
+        up, ep, bp = _boost(u, e, b, v)
+        _, chi64p = orc.omega_sync_fp64(up, ep, bp, B0)
+        gp = np.sqrt(1.0 + up @ up)
+        assert abs(gp * chi64p / (g * chi64) - 1) < 1e-9
 
 
 def test_sync_single_particle_closed_form(orc):
@@ -182,6 +280,52 @@ def test_sync_single_particle_closed_form(orc):
                     for e in bins], np.float32)
     assert np.array_equal(S, ref)
     assert np.count_nonzero(S) > 100
+
+
+def test_sync_adjoint_fp64_complex_step(orc):
+    """The oracle's FP64 adjoint (chain rule through the segment slopes and
+    OmegaSync) equals complex-step derivatives Im L_p(theta + i h) / h of each
+    particle's loss term, which use none of its formulas: every component of
+    dL/dw, dL/du, dL/dB and dL/dE to 1e-9 of the particle's gradient norm, with
+    an electric field, weights and non-unit B0.  Particles with a pair within
+    1e-3 segment of a table node (where the slope jumps and a one-sided
+    derivative is ambiguous) are left out; at least 80% remain."""
+    from ragnar_amd import workloads
+
+    n, nb, B0 = 300, 96, 1.75  # B0 exact in float: the spectrum entry point takes a float
+    u, b = workloads.particles(n, seed=31)
+    rng = np.random.default_rng(32)
+    e = rng.normal(0, 0.2, (3, n)).astype(np.float32)
+    w = rng.uniform(0.5, 2.0, n).astype(np.float32)
+    bins = orc.logspace(1e-2, 1e7, nb)
+    gbar = rng.normal(0, 1, nb)
+    fx, fy = orc.ftable()
+    gw, gu, gb, ge, dn = orc.sync_adjoint_fp64(u, b, bins, gbar, e=e, w=w, B0=B0)
+    keep = dn > 1e-3
+    assert keep.mean() > 0.8
+    h = 1e-30
+    args = [a.astype(np.complex128) for a in (u, e, b, w)]
+    refs = {}
+    for name, idx in (("w", 3), ("u", 0), ("b", 2), ("e", 1)):
+        comps = []
+        for c in range(1 if name == "w" else 3):
+            a = [x.copy() for x in args]
+            if name == "w":
+                a[idx] += 1j * h
+            else:
+                a[idx][c] += 1j * h
+            comps.append(loss_per_particle_complex(a[0], a[1], a[2], a[3], bins, fx, fy, gbar, B0).imag / h)
+        refs[name] = np.array(comps)
+    for name, got in (("w", gw[None, :]), ("u", gu), ("b", gb), ("e", ge)):
+        ref = refs[name]
+        norm = np.linalg.norm(ref, axis=0)
+        err = np.linalg.norm(got - ref, axis=0)[keep] / np.maximum(norm[keep], 1e-300)
+        assert err.max() < 1e-9, (name, err.max())
+        assert np.count_nonzero(norm[keep]) > 0.8 * keep.sum()
+    # the weight gradient is the spectrum's own contraction: dL/dw_p = L_p / w_p
+    S1 = orc.sync_spectrum(u[:, :1], b[:, :1], bins, e=e[:, :1], w=np.ones(1, np.float32), B0=B0,
+                           mode=orc.FP64, want_fp64=True)[1]
+    assert abs(gw[0] - gbar @ S1) <= 1e-12 * np.abs(gbar) @ np.abs(S1)
 
 
 def test_sync_invariants_and_modes(orc):
