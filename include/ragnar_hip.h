@@ -14,8 +14,12 @@
  * rgn_sync(stream).
  *
  * The library works on one device (rgn_init) and keeps one set of scratch
- * buffers on it; compute calls that overlap on different streams share them
- * and race, so callers order such calls (one stream, or events).
+ * buffers on it.  Compute calls that use them are ordered by the library
+ * whatever streams they are issued on: a call on another stream than the
+ * previous such call first waits (hipStreamWaitEvent) for that call's
+ * kernels, so calls on different streams run one after another on the
+ * device, and host threads issue them one at a time.  Calls on one stream
+ * are ordered by the stream itself.
  *
  * Reference interfaces replaced are cited as path:line in haykh/ragnar.
  */
@@ -182,14 +186,35 @@ int rgn_sync_fromdist(const rgn_sync_plan* plan, const float* d_gbeta, const flo
 
 /* Batched contraction S[d, e] = sum_g f[d, g] K[g, e] for nd distributions
  * sharing one gbeta grid (row-major d_f[nd * ng], d_out[nd * nbins]).
- * precision: RGN_PREC_FP32 (FP32 FMA, reference terms), RGN_PREC_F16X3
- * (split-FP16 MFMA, 3 products, FP32 accumulation), RGN_PREC_F16X1 (one FP16
- * MFMA pass; reported, not parity).  No reference counterpart: equals
- * rgn_sync_fromdist per row. */
-enum { RGN_PREC_FP32 = 0, RGN_PREC_F16X3 = 1, RGN_PREC_F16X1 = 2 };
+ * K[g, e] is KernelFromDist's term (synchrotron.hpp:78-93) built on the GPU.
+ * precision: RGN_PREC_FP32 (the library GEMM: rocBLAS SGEMM, FP32
+ * throughout), RGN_PREC_F16X3 (the hand-written split-FP16 MFMA kernel: each
+ * operand split into two fp16 parts with power-of-two scales per block of 64
+ * g, three products, FP32 accumulation; more accurate than the FP32 GEMM),
+ * RGN_PREC_F16X1 (one FP16 product; reported, not parity), RGN_PREC_BF16X6
+ * (the hand-written split-BF16 MFMA kernel: each operand split exactly into
+ * three bf16 parts, which need no scaling, six products, FP32 accumulation;
+ * the dropped products are below ~2^-24 relative).  No reference
+ * counterpart: equals rgn_sync_fromdist per row. */
+enum { RGN_PREC_FP32 = 0, RGN_PREC_F16X3 = 1, RGN_PREC_F16X1 = 2, RGN_PREC_BF16X6 = 3 };
 int rgn_sync_fromdist_batched(const rgn_sync_plan* plan, const float* d_gbeta, int ng,
                               int dist_log, const float* d_f, size_t nd, int precision,
                               float* d_out, rgn_stream_t stream);
+
+/* rgn_sync_fromdist_batched with the rows given by two numbers each instead
+ * of ng values: f[d, g] = x_g^p_d exp(-x_g / xc_d) / N_d, x_g = d_gbeta[g],
+ * p_d = d_index[d], xc_d = d_cutoff[d], N_d the trapezoid integral of the
+ * row over the grid (sum_g (f_g + f_g+1) (x_g+1 - x_g) / 2).  The rows are
+ * generated inside the split-FP16 kernel (FP64 exponent, float values), so
+ * the call reads 8 bytes per distribution instead of 4 ng.  precision:
+ * RGN_PREC_F16X3, RGN_PREC_F16X1 or RGN_PREC_BF16X6 (RGN_ERR_UNSUPPORTED
+ * otherwise); ng >= 2.
+ * No reference counterpart (the reference evaluates a generator on the host
+ * per distribution, distributions.cpp:37-66, and integrates each with
+ * SynchrotronSpectrumFromDist, synchrotron.cpp:69-105). */
+int rgn_sync_fromdist_batched_plaw(const rgn_sync_plan* plan, const float* d_gbeta, int ng,
+                                   int dist_log, const float* d_index, const float* d_cutoff,
+                                   size_t nd, int precision, float* d_out, rgn_stream_t stream);
 
 /* ---- energy histogram -----------------------------------------------------
  * Replaces Particles<D>::energyDistribution, particles.cpp:189-260: energy =

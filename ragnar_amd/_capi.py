@@ -15,7 +15,7 @@ HEADER_PATH = _HERE.parent / "include" / "ragnar_hip.h"
 
 RGN_OK, RGN_ERR_INVALID, RGN_ERR_RANGE, RGN_ERR_HIP, RGN_ERR_NOMEM, RGN_ERR_UNSUPPORTED = range(6)
 RGN_FTABLE_REFERENCE, RGN_FTABLE_ACCURATE = 0, 1
-RGN_PREC_FP32, RGN_PREC_F16X3, RGN_PREC_F16X1 = 0, 1, 2
+RGN_PREC_FP32, RGN_PREC_F16X3, RGN_PREC_F16X1, RGN_PREC_BF16X6 = 0, 1, 2, 3
 RGN_METHOD_AUTO, RGN_METHOD_DIRECT, RGN_METHOD_BINNED = 0, 1, 2
 
 vp = C.c_void_p
@@ -56,6 +56,7 @@ _SIGNATURES = {
     "rgn_sync_fromdist": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, vp, vp]),
     "rgn_ic_spectrum": (C.c_int, [vp, vp, C.c_int, C.c_int, vp, vp, C.c_int, vp, C.c_int, vp, vp]),
     "rgn_sync_fromdist_batched": (C.c_int, [vp, vp, C.c_int, C.c_int, vp, sz, C.c_int, vp, vp]),
+    "rgn_sync_fromdist_batched_plaw": (C.c_int, [vp, vp, C.c_int, C.c_int, vp, vp, sz, C.c_int, vp, vp]),
     "rgn_hist_plan_create": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(vp)]),
     "rgn_hist_plan_destroy": (C.c_int, [vp]),
     "rgn_hist_plan_edges": (C.c_int, [vp, vp]),

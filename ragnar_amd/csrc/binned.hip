@@ -21,7 +21,8 @@
 //             coalesced, plus one run descriptor per non-empty bucket;
 //   scan      per bucket, its non-empty runs compacted in block order with
 //             their offsets within the bucket;
-//   subhist   each bucket is cut into work items of up to kItem sources; each
+//   subhist   each bucket is cut into work items of up to `item` sources (at
+//             least a bucket block's chunk; 16384 unless RAGNAR_SUBHIST_ITEM); each
 //             item walks its runs and reduces the sources into M + 1
 //             sub-cells in LDS (ds_add_f64);
 //   prefix    per bucket: the items in order, then prefix sums over sub-cells;
@@ -151,10 +152,9 @@ constexpr int kBucketThreads = 1024;       // threads of a bucket block (two per
 constexpr int kPer = 8;                    // particles per thread in the bucket pass
 constexpr int kChunk = kBucketThreads * kPer;  // particles per bucket block
 static_assert(kChunk <= 32768, "run descriptors pack start and length in 16 bits each");
-constexpr unsigned kItem = 16384;          // sources per sub-histogram work item
+constexpr unsigned kItem = 16384;          // sources per sub-histogram work item (default)
 static_assert(kChunk <= kItem, "a run (at most one chunk) must hold at most one work item's first source");
 constexpr int kSubThreads = 512;           // threads of a sub-histogram work item
-constexpr int kBatch = 4;                  // sources per lane with loads in flight together
 constexpr int kWin = 2048;                 // runs per LDS window of a work item
 constexpr int kPrefThreads = 768;          // 3 item groups x 256 sub-cells per prefix block (two blocks per CU)
 constexpr int kRowThreads = 1024;          // one row of run descriptors per scan block
@@ -323,9 +323,10 @@ __global__ void __launch_bounds__(kBucketThreads) binned_bucket(ParticlesDev D, 
 //   cvoff[k * (G + 1) + i]: virtual offset of run i within the bucket
 //                           (cvoff[k * (G + 1) + nrun[k]] = totals[k]);
 //   cidx[k * G + i]:        physical index of the run's first source;
-//   irun[k * IS + t]:       the run holding source t * kItem of the bucket, the
-//                           first source of its work item t (a run is shorter
-//                           than kItem, so it holds at most one such source).
+//   irun[k * IS + t]:       the run holding source t * item of the bucket, the
+//                           first source of its work item t (a run is at most
+//                           one chunk, not longer than an item, so it holds at
+//                           most one such source).
 // One exclusive scan over blocks of (run length, non-empty flag) packed in 64 bits.
 // The descriptors read are zeroed again, so the table needs no clearing pass
 // before the next call.
@@ -334,7 +335,8 @@ __global__ void __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_e
                                                              unsigned* __restrict__ cidx,
                                                              unsigned* __restrict__ nrun,
                                                              unsigned* __restrict__ totals,
-                                                             unsigned* __restrict__ irun, int IS) {
+                                                             unsigned* __restrict__ irun, int IS,
+                                                             unsigned item) {
   __shared__ unsigned long long part[kRowThreads / 64];
   const int k = blockIdx.x;
   unsigned* row = runs + static_cast<size_t>(k) * G;
@@ -369,8 +371,8 @@ __global__ void __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_e
       const unsigned v = static_cast<unsigned>(acc);
       vo[c] = v;
       ci[c] = static_cast<unsigned>(i) * kChunk + run_start(r);
-      const unsigned t = (v + kItem - 1) / kItem;  // first item start at or after v
-      if (t * kItem < v + len) irun[static_cast<size_t>(k) * IS + t] = c;
+      const unsigned t = (v + item - 1) / item;  // first item start at or after v
+      if (t * item < v + len) irun[static_cast<size_t>(k) * IS + t] = c;
       acc += len + (1ull << 32);
       row[i] = 0u;  // leave the run table zeroed for the next call
     }
@@ -392,24 +394,24 @@ __global__ void __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_e
   }
 }
 
-// Work items: bucket k is cut into ceil(totals[k] / kItem) items of kItem
+// Work items: bucket k is cut into ceil(totals[k] / item) items of `item`
 // sources, numbered bucket after bucket.  Both helpers are block-wide (every
 // thread of the NT-thread block calls them; wsum: NT / 64 words of LDS).
-__device__ __forceinline__ unsigned items_of(unsigned total) { return (total + kItem - 1) / kItem; }
+__device__ __forceinline__ unsigned items_of(unsigned total, unsigned item) { return (total + item - 1) / item; }
 
 // Bucket k of item `it` and the bucket's first item; false past the last item
 // (the same answer in every thread).
 template <int NT>
-__device__ __forceinline__ bool item_bucket(const unsigned* __restrict__ totals, int NK, unsigned it, int& k,
-                                            unsigned& first, unsigned* wsum, unsigned* res) {
+__device__ __forceinline__ bool item_bucket(const unsigned* __restrict__ totals, int NK, unsigned it, unsigned item,
+                                            int& k, unsigned& first, unsigned* wsum, unsigned* res) {
   const int per = (NK + NT - 1) / NT;
   const int q0 = min(NK, static_cast<int>(threadIdx.x) * per), q1 = min(NK, q0 + per);
   unsigned c = 0;
-  for (int q = q0; q < q1; ++q) c += items_of(totals[q]);
+  for (int q = q0; q < q1; ++q) c += items_of(totals[q], item);
   unsigned all;
   unsigned acc = block_prefix<unsigned, NT>(c, all, wsum);
   for (int q = q0; q < q1; ++q) {
-    const unsigned m = items_of(totals[q]);
+    const unsigned m = items_of(totals[q], item);
     if (acc <= it && it < acc + m) {
       res[0] = static_cast<unsigned>(q);
       res[1] = acc;
@@ -425,9 +427,10 @@ __device__ __forceinline__ bool item_bucket(const unsigned* __restrict__ totals,
 
 // First item of bucket k (the number of items of buckets before it).
 template <int NT>
-__device__ __forceinline__ unsigned items_before(const unsigned* __restrict__ totals, int k, unsigned* wsum) {
+__device__ __forceinline__ unsigned items_before(const unsigned* __restrict__ totals, int k, unsigned item,
+                                                 unsigned* wsum) {
   unsigned c = 0;
-  for (int q = threadIdx.x; q < k; q += NT) c += items_of(totals[q]);
+  for (int q = threadIdx.x; q < k; q += NT) c += items_of(totals[q], item);
   unsigned all;
   block_prefix<unsigned, NT>(c, all, wsum);
   return all;
@@ -438,12 +441,13 @@ __device__ __forceinline__ unsigned items_before(const unsigned* __restrict__ to
 // LDS; each wave walks a contiguous share of the positions, 64 per step (one
 // per lane), four loads in flight per lane, and reduces the sources into the
 // M + 1 sub-cells with LDS atomics.
+template <int kBatch>
 __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G, const unsigned* __restrict__ totals,
                                                               const unsigned* __restrict__ cvoff,
                                                               const unsigned* __restrict__ cidx,
                                                               const unsigned* __restrict__ nrun,
                                                               const unsigned* __restrict__ irun, int IS,
-                                                              const float2* __restrict__ sortWP,
+                                                              unsigned item, const float2* __restrict__ sortWP,
                                                               double* __restrict__ partial) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int kWaves = kSubThreads / 64;
@@ -459,7 +463,7 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
   const unsigned it = blockIdx.x;
   int k;
   unsigned first;
-  if (!item_bucket<kSubThreads>(totals, P.NK, it, k, first, s_wsum, s_item)) return;  // uniform per block
+  if (!item_bucket<kSubThreads>(totals, P.NK, it, item, k, first, s_wsum, s_item)) return;  // uniform per block
   for (int i = threadIdx.x; i < M1; i += kSubThreads) {
     m0[i] = 0.0;
     m1[i] = 0.0;
@@ -467,8 +471,8 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
   for (int i = threadIdx.x; i < P.M; i += kSubThreads) athr[i] = P.athr[i];
   for (int c = threadIdx.x; c < JQ2; c += kSubThreads) lut[c] = P.sublut[c];
   __syncthreads();
-  const unsigned v0 = (it - first) * kItem;
-  const unsigned v1 = min(totals[k], v0 + kItem);
+  const unsigned v0 = (it - first) * item;
+  const unsigned v1 = min(totals[k], v0 + item);
   const unsigned* vo = cvoff + static_cast<size_t>(k) * (G + 1);
   const unsigned* ci = cidx + static_cast<size_t>(k) * G;
   // first and last run of the item: the runs holding sources v0 and v1 - 1.
@@ -585,7 +589,7 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
 // Pass 4: per bucket, its items summed (three interleaved item groups, combined
 // in a fixed order), then inclusive prefix sums over sub-cells:
 // P0[k][j] = sum_{j' <= j} m0, P1 likewise.
-__global__ void __launch_bounds__(kPrefThreads) binned_prefix(int M, const unsigned* __restrict__ totals,
+__global__ void __launch_bounds__(kPrefThreads) binned_prefix(int M, unsigned item, const unsigned* __restrict__ totals,
                                                               const double* __restrict__ partial,
                                                               double* __restrict__ P0, double* __restrict__ P1) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -597,7 +601,7 @@ __global__ void __launch_bounds__(kPrefThreads) binned_prefix(int M, const unsig
   double* s1 = s0 + M1;
   const int k = blockIdx.x;
   __shared__ unsigned s_wsum[kPrefThreads / 64];
-  const unsigned i0 = items_before<kPrefThreads>(totals, k, s_wsum), i1 = i0 + items_of(totals[k]);
+  const unsigned i0 = items_before<kPrefThreads>(totals, k, item, s_wsum), i1 = i0 + items_of(totals[k], item);
   const int grp = threadIdx.x >> 8, col = threadIdx.x & 255;
   for (int j0 = 0; j0 < M1; j0 += 256) {
     const int j = j0 + col;
@@ -751,20 +755,84 @@ struct GradPtrs {
   float* ge[3];
 };
 
-template <bool HAS_E, bool NT>
-__global__ void __launch_bounds__(kThreads) binned_adj_gather(ParticlesDev D, size_t n, BinnedDev P,
-                                                              const double2* __restrict__ U, GradPtrs G) {
+// One U entry per particle.  With STAGE, a persistent block first bins the
+// buckets of its first kGather particles, then copies the U rows of the
+// kStage-row window holding most of them into LDS (a power-law population
+// puts most particles in a few consecutive buckets); particles in the window
+// read their entry from LDS, the rest from the table in memory (L2).  The
+// random 16-byte reads from L2 otherwise move a whole cache line each.
+constexpr int kGather = 512;                 // threads of a staged gather block
+constexpr size_t kStageBytes = 60 * 1024;    // LDS for the staged U rows
+
+template <bool HAS_E, bool NT, bool STAGE>
+__global__ void __launch_bounds__(STAGE ? kGather : kThreads) binned_adj_gather(ParticlesDev D, size_t n, BinnedDev P,
+                                                                              const double2* __restrict__ U,
+                                                                              GradPtrs G, int kstage) {
+  constexpr int NT_ = STAGE ? kGather : kThreads;
   extern __shared__ __align__(16) unsigned char smem[];
-  double* alpha = reinterpret_cast<double*>(smem);
+  double2* urows = reinterpret_cast<double2*>(smem);  // STAGE: kstage rows of M + 1 (or the histogram)
+  double* alpha = reinterpret_cast<double*>(smem + (STAGE ? kStageBytes : 0));
   unsigned short* jlut = reinterpret_cast<unsigned short*>(alpha + P.M);
-  for (int i = threadIdx.x; i < P.M; i += kThreads) alpha[i] = P.alpha[i];
-  for (int i = threadIdx.x; i < P.JQ; i += kThreads) jlut[i] = P.jlut[i];
-  __syncthreads();
+  __shared__ int s_w0;
+  for (int i = threadIdx.x; i < P.M; i += NT_) alpha[i] = P.alpha[i];
+  for (int i = threadIdx.x; i < P.JQ; i += NT_) jlut[i] = P.jlut[i];
   const double dlnx = P.K / 0.69314718055994530942;  // dt / dln x
   // n < 2^29 (the factorised form's limit): 32-bit indices and byte offsets
   const unsigned nn = static_cast<unsigned>(n);
-  const unsigned stride = gridDim.x * kThreads;
-  unsigned p = blockIdx.x * kThreads + threadIdx.x;
+  const unsigned stride = gridDim.x * NT_;
+  const size_t M1 = static_cast<size_t>(P.M) + 1;
+  int w0 = 0, w1 = 0;  // staged buckets [w0, w1)
+  if (STAGE) {
+    unsigned* hist = reinterpret_cast<unsigned*>(smem);
+    for (int i = threadIdx.x; i < P.NK; i += NT_) hist[i] = 0u;
+    __syncthreads();
+    const unsigned p = blockIdx.x * NT_ + threadIdx.x;
+    if (p < nn) {
+      const RawParticle r = load_particle32<false>(D, p);
+      float W;
+      double B;
+      if (particle_wb(omega_terms_approx<HAS_E>(r, P.em), r, P, W, B)) {
+        atomicAdd(hist + (static_cast<int>(floor(B)) - P.kmin), 1u);
+      }
+    }
+    __syncthreads();
+    // the window of kstage buckets with the most sampled particles (lowest start on ties)
+    const int nw = max(1, P.NK - kstage + 1);
+    unsigned best = 0;
+    int bw = 0x7fffffff;
+    for (int w = threadIdx.x; w < nw; w += NT_) {
+      unsigned c = 0;
+      for (int q = w; q < min(P.NK, w + kstage); ++q) c += hist[q];
+      if (c > best || (c == best && w < bw)) {
+        best = c;
+        bw = w;
+      }
+    }
+    // block arg-max: pack (count, -start) into 64 bits
+    unsigned long long key = (static_cast<unsigned long long>(best) << 32) |
+                             static_cast<unsigned>(0x7fffffff - min(bw, 0x7fffffff));
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) {
+      const unsigned long long o = __shfl_xor(key, m);
+      key = o > key ? o : key;
+    }
+    __shared__ unsigned long long s_key[NT_ / 64];
+    if ((threadIdx.x & 63) == 0) s_key[threadIdx.x >> 6] = key;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long k2 = s_key[0];
+      for (int i = 1; i < NT_ / 64; ++i) k2 = s_key[i] > k2 ? s_key[i] : k2;
+      s_w0 = 0x7fffffff - static_cast<int>(k2 & 0xffffffffu);
+    }
+    __syncthreads();  // the histogram is read: its LDS takes the rows
+    w0 = min(s_w0, nw - 1);
+    w1 = min(P.NK, w0 + kstage);
+    const size_t nrow = static_cast<size_t>(w1 - w0) * M1;
+    const double2* src = U + static_cast<size_t>(w0) * M1;
+    for (size_t i = threadIdx.x; i < nrow; i += NT_) urows[i] = src[i];
+  }
+  __syncthreads();
+  unsigned p = blockIdx.x * NT_ + threadIdx.x;
   // NT: particles and gradients stream through once (non-temporal), so that
   // the table lookups keep the caches
   RawParticle raw{};
@@ -781,7 +849,8 @@ __global__ void __launch_bounds__(kThreads) binned_adj_gather(ParticlesDev D, si
       const double phi = B - fl;
       const int kk = static_cast<int>(fl) - P.kmin;
       const int j = subcell(alpha, jlut, P.M, P.JQ, phi);
-      const double2 u = U[static_cast<size_t>(kk) * (P.M + 1) + j];
+      const double2 u = (STAGE && kk >= w0 && kk < w1) ? urows[static_cast<size_t>(kk - w0) * M1 + j]
+                                                       : U[static_cast<size_t>(kk) * M1 + j];
       // sum_e g'_e F and sum_e g'_e dF/dln x at this particle
       g = emission_grads(o, cur, u.x - u.y * phi, dlnx * u.y, P.em);
     }
@@ -830,16 +899,18 @@ void sync_forward_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, 
   const int G = static_cast<int>(ceil_div(n, kChunk));
   const size_t NK = plan.NK;
   const size_t M1 = plan.M + 1;
-  // items: sum over k of ceil(totals_k / kItem) <= n / kItem + NK
-  const size_t max_items = n / kItem + NK + 1;
+  // work-item size (sources), at least a bucket block's chunk
+  const unsigned item = static_cast<unsigned>(std::max(env_int("RAGNAR_SUBHIST_ITEM", static_cast<int>(kItem)), kChunk));
+  // items: sum over k of ceil(totals_k / item) <= n / item + NK
+  const size_t max_items = n / item + NK + 1;
   float2* sortWP = static_cast<float2*>(scratch(sizeof(float2) * G * size_t{kChunk}, 6));
   bool fresh = false;
   unsigned* runs = static_cast<unsigned*>(scratch(sizeof(unsigned) * NK * G, 8, &fresh));
   unsigned* cvoff = static_cast<unsigned*>(scratch(sizeof(unsigned) * NK * (G + 1), 10));
   unsigned* cidx = static_cast<unsigned*>(scratch(sizeof(unsigned) * NK * G, 9));
   unsigned* small = static_cast<unsigned*>(scratch(sizeof(unsigned) * (2 * (NK + 1) + 64), 11));
-  // first runs of the work items: a bucket holds at most ceil(G kChunk / kItem) items
-  const int IS = static_cast<int>(ceil_div(size_t{kChunk} * G, kItem));
+  // first runs of the work items: a bucket holds at most ceil(G kChunk / item) items
+  const int IS = static_cast<int>(ceil_div(size_t{kChunk} * G, item));
   unsigned* irun = static_cast<unsigned*>(scratch(sizeof(unsigned) * NK * IS, 14));
   unsigned* totals = small;
   unsigned* nrun = small + (NK + 1);
@@ -870,15 +941,21 @@ void sync_forward_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, 
   }
   RGN_HIP(hipGetLastError());
   stage_mark(st, "binned_bucket");
-  binned_scan_rows<<<plan.NK, kRowThreads, 0, st>>>(runs, G, cvoff, cidx, nrun, totals, irun, IS);
+  binned_scan_rows<<<plan.NK, kRowThreads, 0, st>>>(runs, G, cvoff, cidx, nrun, totals, irun, IS, item);
   RGN_HIP(hipGetLastError());
   runs_clean = clean_after;
   stage_mark(st, "binned_scan_rows");
-  binned_subhist<<<static_cast<unsigned>(max_items), kSubThreads, subhist_smem(plan), st>>>(
-    P, G, totals, cvoff, cidx, nrun, irun, IS, sortWP, partial);
+  // sources per lane with their loads in flight together
+  if (env_int("RAGNAR_SUBHIST_BATCH", 4) == 8) {
+    binned_subhist<8><<<static_cast<unsigned>(max_items), kSubThreads, subhist_smem(plan), st>>>(
+      P, G, totals, cvoff, cidx, nrun, irun, IS, item, sortWP, partial);
+  } else {
+    binned_subhist<4><<<static_cast<unsigned>(max_items), kSubThreads, subhist_smem(plan), st>>>(
+      P, G, totals, cvoff, cidx, nrun, irun, IS, item, sortWP, partial);
+  }
   RGN_HIP(hipGetLastError());
   stage_mark(st, "binned_subhist");
-  binned_prefix<<<plan.NK, kPrefThreads, sizeof(double) * 2 * M1, st>>>(plan.M, totals, partial, P0, P1);
+  binned_prefix<<<plan.NK, kPrefThreads, sizeof(double) * 2 * M1, st>>>(plan.M, item, totals, partial, P0, P1);
   RGN_HIP(hipGetLastError());
   stage_mark(st, "binned_prefix");
   binned_evaluate<<<plan.nbins, kThreads, 0, st>>>(P, P0, P1, out);
@@ -907,10 +984,7 @@ void sync_adjoint_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, 
   }
   BinnedDev Pg = P;
   if (plan.NK == 0) Pg.NK = 0;  // every particle falls outside: gradients are zero
-  const size_t bpc = static_cast<size_t>(env_int("RAGNAR_GATHER_BPC", 8));  // blocks per CU
-  const int grid = static_cast<int>(std::max<size_t>(1, std::min<size_t>(ceil_div(n, kThreads),
-                                                                            static_cast<size_t>(num_cus()) * bpc)));
-  const size_t smem = sizeof(double) * std::max(plan.M, 1) + sizeof(unsigned short) * plan.JQ;
+  const size_t tables = sizeof(double) * std::max(plan.M, 1) + sizeof(unsigned short) * plan.JQ;
   // The general form also for stores without E: the shorter one measured ~4%
   // slower here (more resident waves, same memory traffic).  Non-temporal
   // streams pay off once a call's particles and gradients outgrow the
@@ -918,10 +992,29 @@ void sync_adjoint_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, 
   // just before and outputs read just after are served from it.
   const size_t streams = 6 + (D.e[0] ? 3 : 0) + (D.w ? 1 : 0) + (G.gw ? 1 : 0) + (G.gu[0] ? 3 : 0) +
                          (G.gb[0] ? 3 : 0) + (G.ge[0] ? 3 : 0);
-  if (sizeof(float) * streams * n > (size_t{256} << 20)) {
-    binned_adj_gather<true, true><<<grid, kThreads, smem, st>>>(D, n, Pg, U, G);
+  const bool nt = sizeof(float) * streams * n > (size_t{256} << 20);
+  // staged rows: as many as kStageBytes holds (and the histogram of NK buckets fits)
+  const int kstage = static_cast<int>(std::min<size_t>(plan.NK, kStageBytes / (sizeof(double2) * (plan.M + 1))));
+  const bool stage = Pg.NK > 0 && kstage >= 1 && sizeof(unsigned) * plan.NK <= kStageBytes &&
+                     env_int("RAGNAR_GATHER_STAGE", 1) == 1;
+  if (stage) {
+    const int grid = static_cast<int>(std::max<size_t>(
+      1, std::min<size_t>(ceil_div(n, kGather), static_cast<size_t>(num_cus()) * env_int("RAGNAR_GATHER_SBPC", 2))));
+    const size_t smem = kStageBytes + tables;
+    if (nt) {
+      binned_adj_gather<true, true, true><<<grid, kGather, smem, st>>>(D, n, Pg, U, G, kstage);
+    } else {
+      binned_adj_gather<true, false, true><<<grid, kGather, smem, st>>>(D, n, Pg, U, G, kstage);
+    }
   } else {
-    binned_adj_gather<true, false><<<grid, kThreads, smem, st>>>(D, n, Pg, U, G);
+    const size_t bpc = static_cast<size_t>(env_int("RAGNAR_GATHER_BPC", 8));  // blocks per CU
+    const int grid = static_cast<int>(std::max<size_t>(1, std::min<size_t>(ceil_div(n, kThreads),
+                                                                              static_cast<size_t>(num_cus()) * bpc)));
+    if (nt) {
+      binned_adj_gather<true, true, false><<<grid, kThreads, tables, st>>>(D, n, Pg, U, G, 0);
+    } else {
+      binned_adj_gather<true, false, false><<<grid, kThreads, tables, st>>>(D, n, Pg, U, G, 0);
+    }
   }
   RGN_HIP(hipGetLastError());
   stage_mark(st, "binned_adj_gather");

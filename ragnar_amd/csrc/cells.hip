@@ -11,9 +11,11 @@
 // b_lo(c)..b_hi(c) in order: the result is deterministic and equals the
 // per-cell spectrum.
 //
-// Tuning.  The block count G (blocks per CU) is chosen by timing candidates on
-// first use per shape class and cached in a JSON file (RAGNAR_TUNE_CACHE,
-// default ~/.cache/ragnar_amd/tune.json; RAGNAR_TUNE=0 disables tuning).
+// Tuning.  The launch shape (blocks per CU in {1, 2, 4, 8, 16}, bins split
+// over one or two waves) is chosen by timing every candidate on first use per
+// shape class (device, bins, log2 of the cell and particle counts) and cached
+// in a JSON file (RAGNAR_TUNE_CACHE, default ~/.cache/ragnar_amd/tune.json;
+// RAGNAR_TUNE=0 disables tuning: 4 blocks per CU, no split).
 // RAGNAR_CELLS_GRAPH=0 launches without the graph.
 #include <algorithm>
 #include <cstdio>
@@ -24,6 +26,7 @@
 #include <sstream>
 #include <string>
 #include <sys/stat.h>
+#include <unistd.h>
 #include <vector>
 
 #include "emission.hpp"
@@ -34,10 +37,18 @@ namespace {
 
 constexpr int kThreads = 256;
 
-template <int NB>
+// SPLIT = 1: each wave takes 64 of a tile's 256 particles and all 64 NB bins
+// (NB per lane).  SPLIT = 2: waves 2 g + h take particles [128 g, 128 g + 128)
+// and bin half h (NB / 2 per lane): half the accumulators per lane, twice the
+// particle reads per bin.
+template <int NB, int SPLIT>
 __global__ void __launch_bounds__(kThreads) cells_kernel(ParticlesDev D, PlanDev P,
                                                          const int64_t* __restrict__ off, int ncells,
                                                          int64_t Q, double* __restrict__ records) {
+  static_assert(NB % SPLIT == 0, "bins per lane must split evenly");
+  constexpr int NBL = NB / SPLIT;              // bins per lane
+  constexpr int kPW = 64 * SPLIT;              // particles per wave and tile
+  constexpr int kGroups = (kThreads / 64) / SPLIT;
   extern __shared__ __align__(16) unsigned char smem[];
   float4* tile = reinterpret_cast<float4*>(smem);
   double* red = reinterpret_cast<double*>(smem + kThreads * sizeof(float4));
@@ -46,10 +57,11 @@ __global__ void __launch_bounds__(kThreads) cells_kernel(ParticlesDev D, PlanDev
 
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  float c[NB], S[NB];
+  const int h = wave % SPLIT, grp = wave / SPLIT;
+  float c[NBL], S[NBL];
 #pragma unroll
-  for (int k = 0; k < NB; ++k) {
-    const float2 cs = P.bincs[lane + 64 * k];
+  for (int k = 0; k < NBL; ++k) {
+    const float2 cs = P.bincs[lane + 64 * (h * NBL + k)];
     c[k] = cs.x;
     S[k] = cs.y;
   }
@@ -66,9 +78,9 @@ __global__ void __launch_bounds__(kThreads) cells_kernel(ParticlesDev D, PlanDev
   for (int cell = lo; cell < ncells && off[cell] < p1; ++cell) {
     const int64_t s0 = max(p0, off[cell]), s1 = min(p1, off[cell + 1]);
     if (s0 >= s1) continue;  // empty cell: no record
-    double dacc[NB];
+    double dacc[NBL];
 #pragma unroll
-    for (int k = 0; k < NB; ++k) dacc[k] = 0.0;
+    for (int k = 0; k < NBL; ++k) dacc[k] = 0.0;
     for (int64_t base = s0; base < s1; base += kThreads) {
       const int64_t p = base + threadIdx.x;
       Source s{0.0f, P.v_safe, 0.0f};
@@ -76,31 +88,33 @@ __global__ void __launch_bounds__(kThreads) cells_kernel(ParticlesDev D, PlanDev
       __syncthreads();
       tile[threadIdx.x] = make_float4(s.W, s.vhi, s.vlo, 0.0f);
       __syncthreads();
-      float acc[NB];
+      float acc[NBL];
 #pragma unroll
-      for (int k = 0; k < NB; ++k) acc[k] = 0.0f;
-      const float4* src = tile + wave * 64;
+      for (int k = 0; k < NBL; ++k) acc[k] = 0.0f;
+      const float4* src = tile + grp * kPW;
 #pragma unroll 4
-      for (int j = 0; j < 64; ++j) {
+      for (int j = 0; j < kPW; ++j) {
         const float4 q = src[j];
 #pragma unroll
-        for (int k = 0; k < NB; ++k) {
+        for (int k = 0; k < NBL; ++k) {
           const float u = fmaf(c[k], S[k], q.y) + q.z;
           const float2 cs = ltab[cvt_sat_u32(u)];
           acc[k] = fmaf(q.x, fmaf(u, cs.y, cs.x), acc[k]);
         }
       }
 #pragma unroll
-      for (int k = 0; k < NB; ++k) dacc[k] += static_cast<double>(acc[k]);
+      for (int k = 0; k < NBL; ++k) dacc[k] += static_cast<double>(acc[k]);
     }
+    // red[(g * NB + bin block) * 64 + lane]: the partial of particle group g
 #pragma unroll
-    for (int k = 0; k < NB; ++k) red[(wave * NB + k) * 64 + lane] = dacc[k];
+    for (int k = 0; k < NBL; ++k) red[(grp * NB + h * NBL + k) * 64 + lane] = dacc[k];
     __syncthreads();
     double* rec = records + static_cast<size_t>(cell + blockIdx.x) * (64 * NB);
     for (int e = threadIdx.x; e < 64 * NB; e += kThreads) {
-      const int k = e >> 6, l = e & 63;
-      rec[e] = ((red[(0 * NB + k) * 64 + l] + red[(1 * NB + k) * 64 + l]) + red[(2 * NB + k) * 64 + l]) +
-               red[(3 * NB + k) * 64 + l];
+      double t = red[e];
+#pragma unroll
+      for (int g = 1; g < kGroups; ++g) t += red[g * NB * 64 + e];  // fixed order
+      rec[e] = t;
     }
     __syncthreads();  // red is reused by the next segment
   }
@@ -133,22 +147,27 @@ size_t cells_smem(const SyncPlan& plan) {
 }
 
 void launch_cells(const SyncPlan& plan, const ParticlesDev& D, const int64_t* d_off, int ncells,
-                  int64_t Q, int grid, double* records, float* out, hipStream_t st) {
+                  int64_t Q, int grid, int split, double* records, float* out, hipStream_t st) {
   const PlanDev P = device_view(plan);
   const size_t smem = cells_smem(plan);
-  switch (plan.NB) {
-#define RGN_CELLS_CASE(NB_)                                                                  \
-  case NB_:                                                                                  \
-    cells_kernel<NB_><<<grid, kThreads, smem, st>>>(D, P, d_off, ncells, Q, records);       \
+  RGN_REQUIRE(split == 1 || (split == 2 && plan.NB % 2 == 0), RGN_ERR_INVALID, "bin split must divide the bins");
+  switch (plan.NB * 4 + split) {
+#define RGN_CELLS_CASE(NB_, SP_)                                                             \
+  case NB_ * 4 + SP_:                                                                        \
+    cells_kernel<NB_, SP_><<<grid, kThreads, smem, st>>>(D, P, d_off, ncells, Q, records);  \
     break;
-    RGN_CELLS_CASE(1)
-    RGN_CELLS_CASE(2)
-    RGN_CELLS_CASE(3)
-    RGN_CELLS_CASE(4)
-    RGN_CELLS_CASE(5)
-    RGN_CELLS_CASE(6)
-    RGN_CELLS_CASE(7)
-    RGN_CELLS_CASE(8)
+    RGN_CELLS_CASE(1, 1)
+    RGN_CELLS_CASE(2, 1)
+    RGN_CELLS_CASE(2, 2)
+    RGN_CELLS_CASE(3, 1)
+    RGN_CELLS_CASE(4, 1)
+    RGN_CELLS_CASE(4, 2)
+    RGN_CELLS_CASE(5, 1)
+    RGN_CELLS_CASE(6, 1)
+    RGN_CELLS_CASE(6, 2)
+    RGN_CELLS_CASE(7, 1)
+    RGN_CELLS_CASE(8, 1)
+    RGN_CELLS_CASE(8, 2)
 #undef RGN_CELLS_CASE
     default:
       throw Error(RGN_ERR_UNSUPPORTED, "at most 512 photon bins per plan");
@@ -160,11 +179,16 @@ void launch_cells(const SyncPlan& plan, const ParticlesDev& D, const int64_t* d_
 }
 
 // ---------------------------------------------------------------------------
-// Autotuner: blocks per CU for a shape class, cached as JSON lines "key value".
+// Autotuner: launch shape (blocks per CU, bin split) per shape class, cached
+// in a JSON file of "key": value lines.  The key names the device (architecture
+// and CU count); values are checked against the candidates when read, so a
+// damaged or foreign file costs a re-tune, never a bad launch.  The file is
+// replaced by rename(), so a concurrent reader sees the old or the new file.
 // ---------------------------------------------------------------------------
 std::mutex g_tune_mutex;
 std::map<std::string, int> g_tune;
 bool g_tune_loaded = false;
+constexpr int kBpcCandidates[] = {1, 2, 4, 8, 16};
 
 std::string tune_path() {
   if (const char* p = std::getenv("RAGNAR_TUNE_CACHE")) return p;
@@ -177,10 +201,13 @@ void tune_load() {
   g_tune_loaded = true;
   std::ifstream in(tune_path());
   std::string line;
-  // format: {"key": value, ...} written one entry per line
   while (std::getline(in, line)) {
-    const auto q1 = line.find('"'), q2 = line.find('"', q1 + 1), colon = line.find(':', q2);
-    if (q1 == std::string::npos || q2 == std::string::npos || colon == std::string::npos) continue;
+    const auto q1 = line.find('"');
+    if (q1 == std::string::npos) continue;
+    const auto q2 = line.find('"', q1 + 1);
+    if (q2 == std::string::npos) continue;
+    const auto colon = line.find(':', q2);
+    if (colon == std::string::npos) continue;
     g_tune[line.substr(q1 + 1, q2 - q1 - 1)] = std::atoi(line.c_str() + colon + 1);
   }
 }
@@ -194,13 +221,31 @@ void tune_store() {
       if (i == dir.size() || dir[i] == '/') ::mkdir(dir.substr(0, i).c_str(), 0755);
     }
   }
-  std::ofstream out(path);
-  out << "{\n";
-  size_t i = 0;
-  for (const auto& kv : g_tune) {
-    out << "  \"" << kv.first << "\": " << kv.second << (++i < g_tune.size() ? ",\n" : "\n");
+  const std::string tmp = path + ".tmp" + std::to_string(static_cast<long>(::getpid()));
+  {
+    std::ofstream out(tmp);
+    out << "{\n";
+    size_t i = 0;
+    for (const auto& kv : g_tune) {
+      out << "  \"" << kv.first << "\": " << kv.second << (++i < g_tune.size() ? ",\n" : "\n");
+    }
+    out << "}\n";
+    if (!out) return;  // a cache that cannot be written only costs a re-tune
   }
-  out << "}\n";
+  if (std::rename(tmp.c_str(), path.c_str()) != 0) std::remove(tmp.c_str());
+}
+
+// The cached shape of a class, if both entries are valid candidates.
+bool tune_lookup(const std::string& key, int nb_per_lane, int& bpc, int& split) {
+  const auto b = g_tune.find(key + "/bpc"), s = g_tune.find(key + "/split");
+  if (b == g_tune.end() || s == g_tune.end()) return false;
+  const bool bpc_ok = std::find(std::begin(kBpcCandidates), std::end(kBpcCandidates), b->second) !=
+                      std::end(kBpcCandidates);
+  const bool split_ok = s->second == 1 || (s->second == 2 && nb_per_lane % 2 == 0);
+  if (!bpc_ok || !split_ok) return false;
+  bpc = b->second;
+  split = s->second;
+  return true;
 }
 
 int log2_bucket(int64_t v) {
@@ -224,6 +269,7 @@ struct GraphKey {
   int ncells;
   int64_t Q;
   int grid;
+  int split;
   bool operator<(const GraphKey& o) const {
     return std::lexicographical_compare(reinterpret_cast<const char*>(this),
                                         reinterpret_cast<const char*>(this) + sizeof(GraphKey),
@@ -250,17 +296,18 @@ static void run_cells(const SyncPlan& plan, const ParticlesDev& D, const int64_t
   int64_t* d_off = static_cast<int64_t*>(scratch(sizeof(int64_t) * (ncells + 1), 4));
   RGN_HIP(hipMemcpyAsync(d_off, h_off, sizeof(int64_t) * (ncells + 1), hipMemcpyHostToDevice, st));
 
-  // tuned blocks per CU for this shape class
+  // tuned launch shape for this shape class
   const char* tune_env = std::getenv("RAGNAR_TUNE");
   const bool tune = !(tune_env && std::string(tune_env) == "0");
   std::ostringstream key;
-  key << "cells:nb" << plan.nbins_pad << ":c" << log2_bucket(ncells) << ":n" << log2_bucket(total + 1);
-  int bpc = 4;
+  key << "cells:" << device_arch() << ":cu" << num_cus() << ":nb" << plan.nbins_pad << ":c"
+      << log2_bucket(ncells) << ":n" << log2_bucket(total + 1);
+  int bpc = 4, split = 1;
+  bool cached = false;
   {
     std::lock_guard<std::mutex> lk(g_tune_mutex);
     tune_load();
-    auto it = g_tune.find(key.str());
-    if (it != g_tune.end()) bpc = it->second;
+    cached = tune_lookup(key.str(), plan.NB, bpc, split);
   }
   auto plan_grid = [&](int blocks_per_cu, int64_t& Q) {
     const int64_t g = std::max<int64_t>(1, std::min<int64_t>(static_cast<int64_t>(num_cus()) * blocks_per_cu,
@@ -271,35 +318,34 @@ static void run_cells(const SyncPlan& plan, const ParticlesDev& D, const int64_t
   const size_t rec_bytes_max = sizeof(double) * plan.nbins_pad * (static_cast<size_t>(ncells) + num_cus() * 16 + 1);
   double* records = static_cast<double*>(scratch(rec_bytes_max, 3));
 
-  bool tuned_now = false;
-  if (tune) {
-    std::lock_guard<std::mutex> lk(g_tune_mutex);
-    tuned_now = g_tune.find(key.str()) == g_tune.end();
-  }
-  if (tuned_now && total > 0) {
+  if (tune && !cached && total > 0) {
     hipEvent_t e0, e1;
     RGN_HIP(hipEventCreate(&e0));
     RGN_HIP(hipEventCreate(&e1));
     float best_ms = 1e30f;
-    for (int cand : {1, 2, 4, 8, 16}) {
-      int64_t Q;
-      const int grid = plan_grid(cand, Q);
-      launch_cells(plan, D, d_off, ncells, Q, grid, records, out, st);  // warm
-      RGN_HIP(hipEventRecord(e0, st));
-      for (int r = 0; r < 3; ++r) launch_cells(plan, D, d_off, ncells, Q, grid, records, out, st);
-      RGN_HIP(hipEventRecord(e1, st));
-      RGN_HIP(hipEventSynchronize(e1));
-      float ms = 0.f;
-      RGN_HIP(hipEventElapsedTime(&ms, e0, e1));
-      if (ms < best_ms) {
-        best_ms = ms;
-        bpc = cand;
+    for (int sp = 1; sp <= (plan.NB % 2 == 0 ? 2 : 1); ++sp) {
+      for (int cand : kBpcCandidates) {
+        int64_t Q;
+        const int grid = plan_grid(cand, Q);
+        launch_cells(plan, D, d_off, ncells, Q, grid, sp, records, out, st);  // warm
+        RGN_HIP(hipEventRecord(e0, st));
+        for (int r = 0; r < 3; ++r) launch_cells(plan, D, d_off, ncells, Q, grid, sp, records, out, st);
+        RGN_HIP(hipEventRecord(e1, st));
+        RGN_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        RGN_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best_ms) {
+          best_ms = ms;
+          bpc = cand;
+          split = sp;
+        }
       }
     }
     RGN_HIP(hipEventDestroy(e0));
     RGN_HIP(hipEventDestroy(e1));
     std::lock_guard<std::mutex> lk(g_tune_mutex);
-    g_tune[key.str()] = bpc;
+    g_tune[key.str() + "/bpc"] = bpc;
+    g_tune[key.str() + "/split"] = split;
     tune_store();
   }
 
@@ -308,7 +354,7 @@ static void run_cells(const SyncPlan& plan, const ParticlesDev& D, const int64_t
   const char* graph_env = std::getenv("RAGNAR_CELLS_GRAPH");
   const bool use_graph = !(graph_env && std::string(graph_env) == "0");
   if (!use_graph) {
-    launch_cells(plan, D, d_off, ncells, Q, grid, records, out, st);
+    launch_cells(plan, D, d_off, ncells, Q, grid, split, records, out, st);
     return;
   }
   GraphKey gk{};
@@ -325,6 +371,7 @@ static void run_cells(const SyncPlan& plan, const ParticlesDev& D, const int64_t
   gk.ncells = ncells;
   gk.Q = Q;
   gk.grid = grid;
+  gk.split = split;
   std::lock_guard<std::mutex> lk(g_graph_mutex);
   auto it = g_graphs.find(gk);
   if (it == g_graphs.end()) {
@@ -332,7 +379,7 @@ static void run_cells(const SyncPlan& plan, const ParticlesDev& D, const int64_t
     RGN_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
     hipGraph_t graph;
     RGN_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-    launch_cells(plan, D, d_off, ncells, Q, grid, records, out, cap);
+    launch_cells(plan, D, d_off, ncells, Q, grid, split, records, out, cap);
     RGN_HIP(hipStreamEndCapture(cap, &graph));
     hipGraphExec_t exec;
     RGN_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
@@ -372,6 +419,7 @@ extern "C" int rgn_sync_cells(const rgn_sync_plan* plan, const float* const d_u[
     RGN_REQUIRE(h_offsets != nullptr, RGN_ERR_INVALID, "null offsets");
     const size_t n = ncells > 0 ? static_cast<size_t>(h_offsets[ncells]) : 0;
     const ParticlesDev D = make_particles_checked(d_u, d_e, d_b, d_w, n);
+    ScratchUse order(as_stream(stream));
     run_cells(plan->p, D, h_offsets, ncells, d_out, as_stream(stream));
   });
 }

@@ -21,6 +21,12 @@ namespace rgn {
 // Split-FP16 MFMA form, contraction_f16.hip.
 void contraction_f16(const SyncPlan& plan, const float* d_k, int ng, const float* d_f, size_t nd, int nprod,
                      float* d_out, hipStream_t st);
+// Split-BF16 form: rows from d_f, or generated when d_index is not null.
+void contraction_bf16(const SyncPlan& plan, const float* d_k, int ng, const float* d_f, const float* d_gbeta,
+                      const float* d_index, const float* d_cutoff, size_t nd, float* d_out, hipStream_t st);
+// The same with the rows generated from (index, cutoff) inside the kernel.
+void contraction_f16_plaw(const SyncPlan& plan, const float* d_k, int ng, const float* d_gbeta, const float* d_index,
+                          const float* d_cutoff, size_t nd, int nprod, float* d_out, hipStream_t st);
 
 namespace {
 
@@ -76,6 +82,7 @@ extern "C" int rgn_sync_fromdist_batched(const rgn_sync_plan* plan, const float*
     RGN_REQUIRE(ng > 0, RGN_ERR_INVALID, "empty distribution grid");
     const SyncPlan& p = plan->p;
     hipStream_t st = as_stream(stream);
+    ScratchUse order(st);
     float* d_k = static_cast<float*>(scratch(sizeof(float) * static_cast<size_t>(ng) * p.nbins, 5));
     build_kernel_matrix(p, d_gbeta, ng, dist_log, d_k, st);
     if (nd == 0) return;
@@ -92,8 +99,34 @@ extern "C" int rgn_sync_fromdist_batched(const rgn_sync_plan* plan, const float*
       RGN_REQUIRE(s == rocblas_status_success, RGN_ERR_HIP, "rocblas_sgemm failed");
     } else if (precision == RGN_PREC_F16X3 || precision == RGN_PREC_F16X1) {
       contraction_f16(p, d_k, ng, d_f, nd, precision == RGN_PREC_F16X3 ? 3 : 1, d_out, st);
+    } else if (precision == RGN_PREC_BF16X6) {
+      contraction_bf16(p, d_k, ng, d_f, d_gbeta, nullptr, nullptr, nd, d_out, st);
     } else {
       throw Error(RGN_ERR_INVALID, "unknown precision mode");
+    }
+  });
+}
+
+extern "C" int rgn_sync_fromdist_batched_plaw(const rgn_sync_plan* plan, const float* d_gbeta, int ng,
+                                              int dist_log, const float* d_index, const float* d_cutoff,
+                                              size_t nd, int precision, float* d_out, rgn_stream_t stream) {
+  return guarded([&] {
+    RGN_REQUIRE(plan != nullptr, RGN_ERR_INVALID, "null plan");
+    RGN_REQUIRE(d_gbeta && d_out && (nd == 0 || (d_index && d_cutoff)), RGN_ERR_INVALID, "null argument");
+    RGN_REQUIRE(ng >= 2, RGN_ERR_INVALID, "a generated distribution needs at least two grid points");
+    RGN_REQUIRE(precision == RGN_PREC_F16X3 || precision == RGN_PREC_F16X1 || precision == RGN_PREC_BF16X6,
+                RGN_ERR_UNSUPPORTED,
+                "generated rows run on the matrix-core kernels only (RGN_PREC_F16X3, _F16X1 or _BF16X6)");
+    const SyncPlan& p = plan->p;
+    hipStream_t st = as_stream(stream);
+    ScratchUse order(st);
+    float* d_k = static_cast<float*>(scratch(sizeof(float) * static_cast<size_t>(ng) * p.nbins, 5));
+    build_kernel_matrix(p, d_gbeta, ng, dist_log, d_k, st);
+    if (precision == RGN_PREC_BF16X6) {
+      contraction_bf16(p, d_k, ng, nullptr, d_gbeta, d_index, d_cutoff, nd, d_out, st);
+    } else {
+      contraction_f16_plaw(p, d_k, ng, d_gbeta, d_index, d_cutoff, nd, precision == RGN_PREC_F16X3 ? 3 : 1, d_out,
+                           st);
     }
   });
 }

@@ -594,6 +594,7 @@ int rgn_sync_spectrum_method(const rgn_sync_plan* plan, const float* const d_u[3
     RGN_REQUIRE(plan != nullptr, RGN_ERR_INVALID, "null plan");
     RGN_REQUIRE(d_out != nullptr, RGN_ERR_INVALID, "null output");
     const ParticlesDev D = make_particles_checked(d_u, d_e, d_b, d_w, n);
+    ScratchUse order(as_stream(stream));
     if (resolve_method(plan->p, n, method, false) == RGN_METHOD_BINNED) {
       sync_forward_binned(plan->p, D, n, d_out, as_stream(stream));
     } else {
@@ -626,6 +627,7 @@ int rgn_sync_adjoint_method(const rgn_sync_plan* plan, const float* const d_u[3]
     RGN_REQUIRE(plan != nullptr, RGN_ERR_INVALID, "null plan");
     RGN_REQUIRE(d_gbar != nullptr, RGN_ERR_INVALID, "null upstream gradient");
     const ParticlesDev D = make_particles_checked(d_u, d_e, d_b, d_w, n);
+    ScratchUse order(as_stream(stream));
     if (resolve_method(plan->p, n, method, true) == RGN_METHOD_BINNED) {
       sync_adjoint_binned(plan->p, D, n, d_gbar, d_gw, d_gu, d_gb, d_ge, as_stream(stream));
       return;
@@ -701,6 +703,7 @@ int rgn_sync_fromdist(const rgn_sync_plan* plan, const float* d_gbeta, const flo
     RGN_REQUIRE(d_gbeta != nullptr && d_f != nullptr && d_out != nullptr, RGN_ERR_INVALID,
                 "null argument");
     RGN_REQUIRE(ng >= 0, RGN_ERR_INVALID, "negative distribution size");
+    ScratchUse order(as_stream(stream));
     const SyncPlan& p = plan->p;
     const int grid = forward_grid(static_cast<size_t>(ng), 4);
     double* partials =

@@ -248,6 +248,55 @@ __global__ void __launch_bounds__(kThreads) hist_combine_kernel(const double* __
   }
 }
 
+// Launch shape of one histogram call.
+struct HistShape {
+  int grid = 1;
+  size_t smem = 0;
+  int qper = 1;
+  int vec = 0;
+};
+
+HistShape hist_shape(const rgn_hist_plan& plan, const float* const d_u[3], size_t n) {
+  HistShape sh;
+  const int nb = plan.nb;
+  auto aligned16 = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; };
+  sh.vec = aligned16(d_u[0]) && aligned16(d_u[1]) && aligned16(d_u[2]);
+  const size_t per_thread = sh.vec ? 8 : 1;
+  const size_t tiles = ceil_div(std::max<size_t>(n, 1), kThreads * per_thread);
+  // lane groups per wave: as many private copies as fit in 48 KB of LDS
+  const size_t per_copy = static_cast<size_t>(nb + 1) * (sizeof(double) + sizeof(unsigned));
+  sh.qper = 4;
+  while (sh.qper > 1 && 4 * sh.qper * per_copy + (nb + 1) * sizeof(double) > 48 * 1024) sh.qper /= 2;
+  sh.smem = 4 * sh.qper * per_copy + (nb + 1) * sizeof(double);
+  // one round of blocks: as many per CU as the LDS holds (at most 8), so the
+  // per-block clearing and reduction of the private copies is paid once
+  const size_t per_cu = std::max<size_t>(1, std::min<size_t>(8, (160 * 1024) / sh.smem));
+  sh.grid = static_cast<int>(std::min<size_t>(tiles, static_cast<size_t>(num_cus()) * per_cu));
+  return sh;
+}
+
+// Bytes of per-block partials a launch of `grid` blocks writes (FP64 sums and
+// uint32 counts per bin and block).  Every entry is written before it is read.
+size_t hist_partial_bytes(int nb, int grid) {
+  return (sizeof(double) + sizeof(unsigned)) * static_cast<size_t>(nb) * grid;
+}
+
+void launch_hist(const rgn_hist_plan& plan, const float* const d_u[3], size_t n, int fourvel,
+                 float* d_hist, uint64_t* d_counts, const HistShape& sh, double* partials,
+                 hipStream_t st) {
+  const int nb = plan.nb;
+  unsigned* cpart = reinterpret_cast<unsigned*>(partials + static_cast<size_t>(nb) * sh.grid);
+  hist_kernel<<<sh.grid, kThreads, sh.smem, st>>>(
+    d_u[0], d_u[1], d_u[2], n, plan.d_edges.as<double>(), nb, plan.emin, plan.emax,
+    plan.approx_scale, plan.log_bins, fourvel ? 1 : 0, sh.vec, sh.qper, cpart, partials);
+  RGN_HIP(hipGetLastError());
+  stage_mark(st, "hist_kernel");
+  hist_combine_kernel<<<nb, kThreads, 0, st>>>(partials, cpart, sh.grid, nb, d_hist,
+                                               reinterpret_cast<unsigned long long*>(d_counts));
+  RGN_HIP(hipGetLastError());
+  stage_mark(st, "hist_combine_kernel");
+}
+
 }  // namespace
 }  // namespace rgn
 
@@ -307,32 +356,10 @@ int rgn_energy_hist(const rgn_hist_plan* plan, const float* const d_u[3], size_t
     RGN_REQUIRE(plan != nullptr, RGN_ERR_INVALID, "null plan");
     RGN_REQUIRE(d_u && d_u[0] && d_u[1] && d_u[2] && d_hist, RGN_ERR_INVALID, "null argument");
     hipStream_t st = as_stream(stream);
-    const int nb = plan->nb;
-    auto aligned16 = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; };
-    const int vec = aligned16(d_u[0]) && aligned16(d_u[1]) && aligned16(d_u[2]);
-    const size_t per_thread = vec ? 8 : 1;
-    const size_t tiles = ceil_div(std::max<size_t>(n, 1), kThreads * per_thread);
-    // lane groups per wave: as many private copies as fit in 48 KB of LDS
-    const size_t per_copy = static_cast<size_t>(nb + 1) * (sizeof(double) + sizeof(unsigned));
-    int qper = 4;
-    while (qper > 1 && 4 * qper * per_copy + (nb + 1) * sizeof(double) > 48 * 1024) qper /= 2;
-    const size_t smem = 4 * qper * per_copy + (nb + 1) * sizeof(double);
-    // one round of blocks: as many per CU as the LDS holds (at most 8), so the
-    // per-block clearing and reduction of the private copies is paid once
-    const size_t per_cu = std::max<size_t>(1, std::min<size_t>(8, (160 * 1024) / smem));
-    const int grid = static_cast<int>(std::min<size_t>(tiles, static_cast<size_t>(num_cus()) * per_cu));
-    const size_t np = static_cast<size_t>(nb) * grid;
-    double* partials = static_cast<double*>(scratch((sizeof(double) + sizeof(unsigned)) * np, 2));
-    unsigned* cpart = reinterpret_cast<unsigned*>(partials + np);
-    hist_kernel<<<grid, kThreads, smem, st>>>(
-      d_u[0], d_u[1], d_u[2], n, plan->d_edges.as<double>(), nb, plan->emin, plan->emax,
-      plan->approx_scale, plan->log_bins, fourvel ? 1 : 0, vec, qper, cpart, partials);
-    RGN_HIP(hipGetLastError());
-    stage_mark(st, "hist_kernel");
-    hist_combine_kernel<<<nb, kThreads, 0, st>>>(partials, cpart, grid, nb, d_hist,
-                                                 reinterpret_cast<unsigned long long*>(d_counts));
-    RGN_HIP(hipGetLastError());
-    stage_mark(st, "hist_combine_kernel");
+    ScratchUse order(st);
+    const HistShape sh = hist_shape(*plan, d_u, n);
+    double* partials = static_cast<double*>(rgn::scratch(hist_partial_bytes(plan->nb, sh.grid), 2));
+    launch_hist(*plan, d_u, n, fourvel, d_hist, d_counts, sh, partials, st);
   });
 }
 

@@ -42,7 +42,17 @@ void define_tabulated(py::module& m) {
          "yfill"_a = 0.0)
     .def(py::init([](const py::array_t<real_t, py::array::forcecast>& x,
                      const py::array_t<real_t, py::array::forcecast>& y, real_t yfill) {
-           return TabulatedFunction<LG>(Array1D<real_t>(x), Array1D<real_t>(y), yfill);
+           // tabulation.cpp:43-61: n = x.size(), then x.at(i) and y.at(i) for i < n,
+           // so pybind11's index checks raise IndexError for a shorter y or an
+           // array of more than one dimension, and a longer y is cut to n
+           const size_t n = static_cast<size_t>(x.size());
+           std::vector<real_t> xs(n), ys(n);
+           for (size_t i = 0; i < n; ++i) {
+             xs[i] = x.at(static_cast<py::ssize_t>(i));
+             ys[i] = y.at(static_cast<py::ssize_t>(i));
+           }
+           return TabulatedFunction<LG>(Array1D<real_t>::from_host(xs.data(), n),
+                                        Array1D<real_t>::from_host(ys.data(), n), yfill);
          }),
          "x"_a, "y"_a, "yfill"_a = 0.0)
     .def("xArr", &TabulatedFunction<LG>::xArr)

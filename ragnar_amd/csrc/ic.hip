@@ -117,6 +117,7 @@ int rgn_ic_spectrum(const float* d_g, const float* d_fg, int ng, int g_log, cons
     RGN_REQUIRE((ng == 0 || (d_g && d_fg)) && (nes == 0 || (d_es && d_fes)), RGN_ERR_INVALID,
                 "null argument");
     hipStream_t st = as_stream(stream);
+    ScratchUse order(st);
     // enough workgroups to fill the device: gamma bins split into chunks
     const int chunks = std::min(std::max(1, (8 * num_cus() + neic - 1) / neic), std::max(ng, 1));
     const int gper = std::max(1, (ng + chunks - 1) / chunks);

@@ -16,6 +16,7 @@ std::mutex g_init_mutex;
 bool g_initialized = false;
 int g_device = 0;
 int g_num_cus = 0;
+std::string g_arch;
 
 struct Scratch {
   void* ptr = nullptr;
@@ -23,6 +24,12 @@ struct Scratch {
 };
 std::mutex g_scratch_mutex;
 std::vector<Scratch> g_scratch;  // indexed by slot
+
+// ScratchUse state: the event the last scratch user recorded and its stream.
+std::recursive_mutex g_order_mutex;
+hipEvent_t g_order_event = nullptr;
+hipStream_t g_order_stream = nullptr;
+bool g_order_pending = false;
 }  // namespace
 
 void set_last_error(const std::string& msg) { g_last_error = msg; }
@@ -48,6 +55,7 @@ static void do_init(int device) {
                 " is " + arch);
   g_device = device;
   g_num_cus = prop.multiProcessorCount;
+  g_arch = arch;
   g_initialized = true;
 }
 
@@ -57,6 +65,8 @@ void ensure_init() {
 }
 
 int num_cus() { return g_num_cus > 0 ? g_num_cus : 256; }
+
+std::string device_arch() { return g_arch; }
 
 int env_int(const char* name, int def) {
   const char* v = std::getenv(name);
@@ -118,6 +128,27 @@ void stage_mark(hipStream_t st, const char* name) {
   ++s->n;
 }
 
+ScratchUse::ScratchUse(hipStream_t st) : lock_(g_order_mutex), st_(st) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  RGN_HIP(hipStreamIsCapturing(st, &cs));
+  if (cs != hipStreamCaptureStatusNone) return;
+  if (!g_order_event) RGN_HIP(hipEventCreateWithFlags(&g_order_event, hipEventDisableTiming));
+  if (g_order_pending && g_order_stream != st) RGN_HIP(hipStreamWaitEvent(st, g_order_event, 0));
+  active_ = true;
+}
+
+ScratchUse::~ScratchUse() {
+  if (!active_) return;
+  if (hipEventRecord(g_order_event, st_) == hipSuccess) {
+    g_order_stream = st_;
+    g_order_pending = true;
+  } else {
+    // the next call on another stream cannot be ordered after this one
+    (void)hipStreamSynchronize(st_);
+    g_order_pending = false;
+  }
+}
+
 void* scratch(size_t bytes, int slot) { return scratch(bytes, slot, nullptr); }
 
 void* scratch(size_t bytes, int slot, bool* fresh) {
@@ -174,6 +205,13 @@ int rgn_finalize(void) {
         if (s.ptr) (void)hipFree(s.ptr);
         s = Scratch{};
       }
+    }
+    {
+      std::lock_guard<std::recursive_mutex> lk3(g_order_mutex);
+      if (g_order_event) (void)hipEventDestroy(g_order_event);
+      g_order_event = nullptr;
+      g_order_stream = nullptr;
+      g_order_pending = false;
     }
     g_initialized = false;
     return RGN_OK;

@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
@@ -71,16 +72,39 @@ struct DeviceBuffer {
 };
 
 // Scratch of the library's device, one buffer per slot, reused by every call
-// (grows on demand; never shrinks).  The mutex only guards the slot table:
-// kernels of calls that overlap on different streams share the buffers and
-// race, so such calls must be ordered by the caller.
+// (grows on demand; never shrinks).  The mutex only guards the slot table;
+// calls that use the buffers are ordered by ScratchUse below.
 void* scratch(size_t bytes, int slot);
 // The same; *fresh is set when the slot was (re)allocated by this call, so its
 // previous contents are gone.
 void* scratch(size_t bytes, int slot, bool* fresh);
 
+// Orders the library calls that use the scratch buffers, whatever streams they
+// are issued on.  While alive it holds a process-wide (recursive) lock, so
+// host threads issue such calls one at a time.  On entry, the stream waits for
+// the event that the previous such call recorded, when that call ran on
+// another stream; on exit (also after an error) the event is recorded on this
+// stream.  So a call never starts on the device before the previous call's
+// kernels are done with the shared buffers, and calls on one stream pay
+// nothing but the record.  A stream under graph capture is left alone: a
+// graph's replays are ordered by the stream that launches them.
+class ScratchUse {
+ public:
+  explicit ScratchUse(hipStream_t st);
+  ~ScratchUse();
+  ScratchUse(const ScratchUse&) = delete;
+  ScratchUse& operator=(const ScratchUse&) = delete;
+
+ private:
+  std::unique_lock<std::recursive_mutex> lock_;
+  hipStream_t st_;
+  bool active_ = false;
+};
+
 // Number of compute units of the current device.
 int num_cus();
+// Architecture name of the library's device (e.g. "gfx950:sramecc+:xnack-").
+std::string device_arch();
 
 // Integer from the environment (launch-shape knobs), or def when unset.
 int env_int(const char* name, int def);

@@ -15,7 +15,7 @@ from . import _capi
 from ._capi import FTable, HistPlan, SyncPlan, check, lib, ptr3
 
 __all__ = ["FTable", "SyncPlan", "HistPlan", "sync_spectrum", "sync_adjoint", "sync_cells",
-           "sync_fromdist", "sync_fromdist_batched", "energy_hist", "particle_terms",
+           "sync_fromdist", "sync_fromdist_batched", "sync_fromdist_batched_plaw", "energy_hist", "particle_terms",
            "spectrum_stages", "adjoint_stages", "SynchrotronSpectrumFn", "synchrotron_spectrum"]
 
 _default_table: FTable | None = None
@@ -59,15 +59,36 @@ def _vec(t: torch.Tensor | None, name: str, n: int):
     return t.data_ptr()
 
 
+def _mat(t: torch.Tensor, name: str, ncols: int):
+    """A row-major (rows, ncols) float32 device matrix."""
+    if not t.is_cuda or t.dtype != torch.float32 or t.dim() != 2 or t.shape[1] != ncols:
+        raise ValueError(f"{name} must be a float32 device tensor of shape (rows, {ncols})")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    return t.data_ptr()
+
+
+def _out(t: torch.Tensor | None, shape, device: torch.device, name: str = "out") -> torch.Tensor:
+    """The caller's output tensor, checked, or a new one: the kernels write
+    exactly prod(shape) floats through its pointer."""
+    shape = tuple(int(s) for s in shape)
+    if t is None:
+        return torch.empty(shape, dtype=torch.float32, device=device)
+    if (not t.is_cuda or t.dtype != torch.float32 or tuple(t.shape) != shape
+            or not t.is_contiguous() or t.device != device):
+        raise ValueError(f"{name} must be a contiguous float32 tensor of shape {shape} on {device}")
+    return t
+
+
 METHODS = {"auto": _capi.RGN_METHOD_AUTO, "direct": _capi.RGN_METHOD_DIRECT,
            "binned": _capi.RGN_METHOD_BINNED}
 
 
 def sync_spectrum(plan: SyncPlan, u, b, w=None, e=None, out=None, method="auto") -> torch.Tensor:
     n = u.shape[1]
-    if out is None:
-        out = torch.empty(plan.nbins, dtype=torch.float32, device=u.device)
-    check(lib().rgn_sync_spectrum_method(plan.handle, _soa(u, "u"),
+    pu = _soa(u, "u")
+    out = _out(out, (plan.nbins,), u.device)
+    check(lib().rgn_sync_spectrum_method(plan.handle, pu,
                                          _soa(e, "e", n) or ptr3(None, None, None),
                                          _soa(b, "b", n), _vec(w, "w", n), n, METHODS[method],
                                          out.data_ptr(), _stream()))
@@ -104,10 +125,10 @@ def spectrum_stages(plan: SyncPlan, u, b, w=None, e=None, out=None, method="auto
     """One forward call timed per kernel launch with HIP events on the current
     stream: [(kernel name, ms), ...] in launch order."""
     n = u.shape[1]
-    if out is None:
-        out = torch.empty(plan.nbins, dtype=torch.float32, device=u.device)
+    pu = _soa(u, "u")
+    out = _out(out, (plan.nbins,), u.device)
     ms, names, k = _stage_buffers()
-    check(lib().rgn_sync_spectrum_stages(plan.handle, _soa(u, "u"),
+    check(lib().rgn_sync_spectrum_stages(plan.handle, pu,
                                          _soa(e, "e", n) or ptr3(None, None, None),
                                          _soa(b, "b", n), _vec(w, "w", n), n, METHODS[method],
                                          out.data_ptr(), _stream(), C.cast(ms, C.c_void_p),
@@ -142,11 +163,13 @@ def sync_cells(plan: SyncPlan, u, b, offsets, w=None, e=None, out=None) -> torch
     offs = np.ascontiguousarray(offsets, np.int64)
     ncells = len(offs) - 1
     n = u.shape[1]
-    if offs[-1] > n:
-        raise ValueError("cell offsets exceed the particle count")
-    if out is None:
-        out = torch.empty(ncells, plan.nbins, dtype=torch.float32, device=u.device)
-    check(lib().rgn_sync_cells(plan.handle, _soa(u, "u"), _soa(e, "e", n) or ptr3(None, None, None),
+    if ncells < 1:
+        raise ValueError("offsets must hold at least two entries")
+    if offs[0] < 0 or offs[-1] > n or np.any(np.diff(offs) < 0):
+        raise ValueError("cell offsets must be non-decreasing within [0, N]")
+    pu = _soa(u, "u")
+    out = _out(out, (ncells, plan.nbins), u.device)
+    check(lib().rgn_sync_cells(plan.handle, pu, _soa(e, "e", n) or ptr3(None, None, None),
                                _soa(b, "b", n), _vec(w, "w", n), offs.ctypes.data, ncells,
                                out.data_ptr(), _stream()))
     return out
@@ -154,10 +177,9 @@ def sync_cells(plan: SyncPlan, u, b, offsets, w=None, e=None, out=None) -> torch
 
 def sync_fromdist(plan: SyncPlan, gbeta, f, dist_log: bool, out=None) -> torch.Tensor:
     ng = gbeta.shape[0]
-    if out is None:
-        out = torch.empty(plan.nbins, dtype=torch.float32, device=gbeta.device)
-    check(lib().rgn_sync_fromdist(plan.handle, _vec(gbeta, "gbeta", ng), _vec(f, "f", ng), ng,
-                                  int(dist_log), out.data_ptr(), _stream()))
+    pg, pf = _vec(gbeta, "gbeta", ng), _vec(f, "f", ng)
+    out = _out(out, (plan.nbins,), gbeta.device)
+    check(lib().rgn_sync_fromdist(plan.handle, pg, pf, ng, int(dist_log), out.data_ptr(), _stream()))
     return out
 
 
@@ -166,24 +188,37 @@ def ic_spectrum(g, fg, g_log: bool, es, fes, eic, out=None) -> torch.Tensor:
     at the energies eic (mc^2) of particles (gamma bins g, values fg; g_log:
     log-spaced bins) scattering soft photons (energies es in mc^2, values fes)."""
     ng, nes, neic = g.shape[0], es.shape[0], eic.shape[0]
-    if out is None:
-        out = torch.empty(neic, dtype=torch.float32, device=eic.device)
-    check(lib().rgn_ic_spectrum(_vec(g, "g", ng), _vec(fg, "fg", ng), ng, int(g_log), _vec(es, "es", nes),
-                                _vec(fes, "fes", nes), nes, _vec(eic, "eic", neic), neic, out.data_ptr(),
-                                _stream()))
+    args = (_vec(g, "g", ng), _vec(fg, "fg", ng), ng, int(g_log), _vec(es, "es", nes),
+            _vec(fes, "fes", nes), nes, _vec(eic, "eic", neic), neic)
+    out = _out(out, (neic,), eic.device)
+    check(lib().rgn_ic_spectrum(*args, out.data_ptr(), _stream()))
     return out
 
 
 def sync_fromdist_batched(plan: SyncPlan, gbeta, f2d, dist_log: bool, precision: int,
                           out=None) -> torch.Tensor:
     ng = gbeta.shape[0]
+    pg = _vec(gbeta, "gbeta", ng)
+    pf = _mat(f2d, "f", ng)
     nd = f2d.shape[0]
-    if f2d.shape[1] != ng or not f2d.is_contiguous() or f2d.dtype != torch.float32:
-        raise ValueError("f must be a contiguous float32 (nd, ng) tensor")
-    if out is None:
-        out = torch.empty(nd, plan.nbins, dtype=torch.float32, device=gbeta.device)
-    check(lib().rgn_sync_fromdist_batched(plan.handle, _vec(gbeta, "gbeta", ng), ng, int(dist_log),
-                                          f2d.data_ptr(), nd, precision, out.data_ptr(), _stream()))
+    out = _out(out, (nd, plan.nbins), gbeta.device)
+    check(lib().rgn_sync_fromdist_batched(plan.handle, pg, ng, int(dist_log), pf, nd, precision,
+                                          out.data_ptr(), _stream()))
+    return out
+
+
+def sync_fromdist_batched_plaw(plan: SyncPlan, gbeta, index, cutoff, dist_log: bool, precision: int,
+                               out=None) -> torch.Tensor:
+    """Spectra of nd power laws with exponential cutoff, f_d(x) = x^index_d
+    exp(-x / cutoff_d) normalised by the trapezoid rule over the gbeta grid,
+    generated inside the split-FP16 kernel (precision RGN_PREC_F16X3 or _F16X1)."""
+    ng = gbeta.shape[0]
+    nd = index.shape[0]
+    pg = _vec(gbeta, "gbeta", ng)
+    pi, pc = _vec(index, "index", nd), _vec(cutoff, "cutoff", nd)
+    out = _out(out, (nd, plan.nbins), gbeta.device)
+    check(lib().rgn_sync_fromdist_batched_plaw(plan.handle, pg, ng, int(dist_log), pi, pc, nd, precision,
+                                               out.data_ptr(), _stream()))
     return out
 
 
