@@ -57,6 +57,49 @@ def test_grids_bit_exact_with_oracle(rg, orc):
     assert np.array_equal(rg.Linspace(-3.0, 7.5, 31).as_array(), orc.linspace(-3.0, 7.5, 31))
 
 
+@pytest.mark.parametrize("log", [True, False])
+def test_tabulated_function(rg, log):
+    """TabulatedFunction / TabulatedFunction_log (tabulation.cpp:19-171): built
+    from numpy or from Array1D_f, the six accessors, the MinMax of unsorted x
+    (tabulation.cpp:84-102), and the range_error -> ValueError checks of
+    tabulation.cpp:104-117 with their messages.  From numpy the reference reads
+    x.at(i), y.at(i) for i < len(x) (tabulation.cpp:43-61): a shorter y is
+    pybind11's IndexError, a longer y is cut to len(x)."""
+    cls = rg.TabulatedFunction_log if log else rg.TabulatedFunction
+    x = np.array([3.0, 0.5, 10.0, 2.0, 7.25], np.float32)
+    y = np.array([1.0, -2.0, 0.25, 4.0, 8.0], np.float32)
+    for tf in (cls(x, y), cls(rg.Array1D_f(x), rg.Array1D_f(y)), cls(x=x, y=y, yfill=0.0)):
+        assert np.array_equal(tf.xArr().as_array(), x)
+        assert np.array_equal(tf.yArr().as_array(), y)
+        assert tf.nPoints() == 5
+        assert tf.yFill() == 0.0
+        assert tf.xMin() == np.float32(0.5) and tf.xMax() == np.float32(10.0)
+    assert cls(x, y, -1.5).yFill() == np.float32(-1.5)
+    assert cls(rg.Array1D_f(x), rg.Array1D_f(y), yfill=3.0).yFill() == 3.0
+    # a longer y from numpy is cut to len(x)
+    tf = cls(x, np.arange(9, dtype=np.float32))
+    assert tf.nPoints() == 5 and np.array_equal(tf.yArr().as_array(), np.arange(5, dtype=np.float32))
+    with pytest.raises(IndexError, match="index 3 is out of bounds for axis 0 with size 3"):
+        cls(x, y[:3])
+    # y.size != x.size (Array1D inputs)
+    with pytest.raises(ValueError, match=r"^y.size != x.size in TabulatedFunction$"):
+        cls(rg.Array1D_f(x), rg.Array1D_f(y[:4]))
+    # xmin >= xmax: constant x, one point, and no points
+    for xs in (np.full(4, 2.0, np.float32), np.ones(1, np.float32), np.zeros(0, np.float32)):
+        with pytest.raises(ValueError, match=r"^xmin >= xmax in TabulatedFunction$"):
+            cls(xs, np.zeros(len(xs), np.float32))
+    # xmin <= 0 only for the log variant
+    xz = np.array([0.0, 1.0, 2.0], np.float32)
+    xn = np.array([-1.0, 1.0, 2.0], np.float32)
+    yz = np.ones(3, np.float32)
+    if log:
+        for xs in (xz, xn):
+            with pytest.raises(ValueError, match=r"^xmin <= 0.0 in Logspace TabulatedFunction$"):
+                cls(xs, yz)
+    else:
+        assert cls(xn, yz).xMin() == -1.0 and cls(xz, yz).xMin() == 0.0
+
+
 def _random_components(rng, names):
     return {k: rng.random(10) for k in names}
 

@@ -94,6 +94,44 @@ def test_particle_terms_bit_exact(orc, table, seed, consts, with_e):
         assert np.array_equal(terms[p, 1], chi, equal_nan=True), (p, terms[p, 1], chi)
 
 
+def test_particle_terms_electric_closed_forms(table):
+    """chiR with E != 0 in closed forms (synchrotron.hpp:190-231), on the GPU:
+    E parallel to u with B = 0 or B parallel to u gives |E| / (gamma B0); E
+    perpendicular to u with B = 0 gives |E| / B0; crossed fields (u along x, E
+    along y, B along z) give |Ey - beta Bz| / B0, zero for the force-free drift
+    Ey = beta Bz.  Inputs are exact floats (integer momenta, so u^2 is exact);
+    the float chiR is held to 1e-6 relative, the drift to 1e-6 |beta B|."""
+    B0 = 0.7
+    cases, refs = [], []
+    for umag in (3.0, 40.0, 2000.0):
+        g = np.sqrt(1.0 + umag * umag)
+        beta = umag / g
+        for em in (np.float32(0.2), np.float32(1.5)):
+            em = float(em)
+            cases.append(((0, 0, umag), (0, 0, -em), (0, 0, 0)))
+            refs.append(em / (g * B0))
+            cases.append(((0, 0, umag), (0, 0, em), (0, 0, 0.8)))
+            refs.append(em / (g * B0))
+            cases.append(((umag, 0, 0), (0, em, 0), (0, 0, 0)))
+            refs.append(em / B0)
+        bz = float(np.float32(1.3))
+        for f in (-1.0, 0.5, 3.0, 1.0):
+            ey = float(np.float32(f * beta * bz))
+            cases.append(((umag, 0, 0), (0, ey, 0), (0, 0, bz)))
+            refs.append(abs(ey - beta * bz) / B0)
+    u = np.array([c[0] for c in cases], np.float32).T.copy()
+    e = np.array([c[1] for c in cases], np.float32).T.copy()
+    b = np.array([c[2] for c in cases], np.float32).T.copy()
+    plan = T.SyncPlan(table, np.logspace(-2, 7, 256).astype(np.float32), B0, 1.0, 1.0)
+    chi = T.particle_terms(plan, dev(u), dev(b), None, dev(e)).cpu().numpy()[:, 1].astype(np.float64)
+    refs = np.array(refs)
+    scale = np.linalg.norm(b, axis=0) * np.linalg.norm(u, axis=0) / np.sqrt(1 + (u * u).sum(0)) / B0
+    drift = refs < 1e-6 * np.maximum(scale, 1e-30)
+    assert drift.sum() == 3
+    assert np.all(np.abs(chi[~drift] / refs[~drift] - 1) < 1e-6), (chi, refs)
+    assert np.all(chi[drift] <= 1e-6 * scale[drift])
+
+
 METHODS = ["direct", "binned"]
 
 
