@@ -374,6 +374,11 @@ def summarize_bench(trace_dir, bench_json):
     print("\n".join(md))
 
 
+def progress(msg):
+    """A progress line on stderr (the JSON result is the only stdout line)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def event_time(fn, steps, warmup, torch):
     """Average ms per call of fn, timed with events on the current (launch) stream."""
     for _ in range(warmup):
@@ -438,7 +443,14 @@ def bench_fit(T, workloads, torch, table, bins):
 
 def bench_cells(T, workloads, torch, table):
     """4096 cells, lognormal counts (median 4000, sigma 1): one persistent launch
-    in a hipGraph against 4096 calls of the single-population form."""
+    in a hipGraph against 4096 calls of the single-population form.  The launch
+    shape is tuned afresh (a new cache file) on the first call; the chosen
+    shape is reported."""
+    import os
+    import tempfile
+
+    tune_file = os.path.join(tempfile.mkdtemp(prefix="ragnar_tune_"), "tune.json")
+    os.environ["RAGNAR_TUNE_CACHE"] = tune_file
     u, b, offs, _, _ = workloads.cells(4096, 4000, 1.0)
     nb = 256
     plan = T.SyncPlan(table, logspace(1e-2, 1e7, nb))
@@ -457,9 +469,20 @@ def bench_cells(T, workloads, torch, table):
     views = [(uc.contiguous(), bc.contiguous()) for uc, bc in views]
     ms_calls = event_time(per_cell, 2, 1, torch)
     total = int(offs[-1])
+    shape = {}
+    try:
+        for line in open(tune_file):
+            if '"' in line and ":" in line:
+                key, val = line.strip().rstrip(",").rsplit(":", 1)
+                shape[key.strip().strip('"').rsplit("/", 1)[-1]] = int(val)
+    except OSError:
+        pass
+    k = 64  # the first cells, for the CPU leg
+    sample = (u[:, :offs[k]].copy(), b[:, :offs[k]].copy(), offs[:k + 1].copy(), total, 4096)
     return {"particles": total, "ms_graph": ms_graph, "ms_per_cell_calls": ms_calls,
             "speedup": ms_calls / ms_graph, "evals_per_s": total * nb / (ms_graph * 1e-3),
-            "frac_fp32_peak": 6 * total * nb / (ms_graph * 1e-3) / PEAK_FP32}
+            "tuned_shape": {"blocks_per_cu": shape.get("bpc"), "bin_split": shape.get("split")},
+            "frac_fp32_peak": 6 * total * nb / (ms_graph * 1e-3) / PEAK_FP32}, sample
 
 
 def bench_contraction(T, workloads, torch, table):
@@ -468,7 +491,7 @@ def bench_contraction(T, workloads, torch, table):
     products).  Returns the timings and, for the checker, 32 sampled rows of each
     output with their inputs."""
     nd, ng, nb = 65536, 512, 512
-    f, _, _ = workloads.distributions(nd, ng)
+    f, pidx, gcut = workloads.distributions(nd, ng)
     g = logspace(1, 1e6, ng)
     bins = logspace(1e-2, 1e7, nb)
     plan = T.SyncPlan(table, bins)
@@ -478,12 +501,37 @@ def bench_contraction(T, workloads, torch, table):
     res = {}
     rows = list(range(0, nd, nd // 32))
     samples = {"g": g, "bins": bins, "f": f[rows], "out": {}}
-    for name, prec in (("fp32", 0), ("f16x3", 1), ("f16x1", 2)):
+    for name, prec in (("fp32", 0), ("f16x3", 1), ("f16x1", 2), ("bf16x6", 3)):
         ms = event_time(lambda: T.sync_fromdist_batched(plan, G, Fm, True, prec, out=out), 10, 2, torch)
         samples["out"][name] = out[rows].cpu().numpy()
         flop = 2.0 * nd * ng * nb
         res[name] = {"ms": ms, "TFLOP_per_s": flop / (ms * 1e-3) / 1e12,
                      "GB_per_s": (nd * ng + nd * nb) * 4 / (ms * 1e-3) / 1e9}
+    # rows generated inside the kernel from (index, cutoff): 8 bytes per row in
+    P = torch.from_numpy(pidx.astype("float32")).cuda()
+    Cc = torch.from_numpy(gcut.astype("float32")).cuda()
+    for name, prec in (("f16x3_generated", 1), ("f16x1_generated", 2), ("bf16x6_generated", 3)):
+        ms = event_time(lambda: T.sync_fromdist_batched_plaw(plan, G, P, Cc, True, prec, out=out), 10, 2, torch)
+        samples["out"][name] = out[rows].cpu().numpy()
+        flop = 2.0 * nd * ng * nb
+        res[name] = {"ms": ms, "TFLOP_per_s": flop / (ms * 1e-3) / 1e12,
+                     "GB_per_s": (8 * nd + 4 * nd * nb) / (ms * 1e-3) / 1e9,
+                     "hbm_floor_ms": (8 * nd + 4 * nd * nb) / PEAK_HBM * 1e3}
+    samples["index"], samples["cutoff"] = pidx[rows], gcut[rows]
+    # the plain per-call form: one SynchrotronSpectrumFromDist call per row
+    # (8192 rows timed, scaled to all rows)
+    k = 8192
+    row_in = [Fm[d] for d in range(k)]
+    row_out = [out[d] for d in range(k)]
+
+    def per_row():
+        for fr, o in zip(row_in, row_out):
+            T.sync_fromdist(plan, G, fr, True, out=o)
+
+    ms_rows = event_time(per_row, 2, 1, torch) * nd / k
+    res["per_row_calls"] = {"ms": ms_rows, "rows_timed": k,
+                            "speedup_f16x3": ms_rows / res["f16x3"]["ms"],
+                            "speedup_fp32": ms_rows / res["fp32"]["ms"]}
     return res, samples
 
 
@@ -537,27 +585,49 @@ def roofline(prof, n, nb, plan):
             "kernel_ms": ms, "algorithmic_bytes": nbytes}
 
 
-def oracle_checks(u, b, bins, S_sub, contraction_samples):
-    """The checker: the timed path's outputs against the oracle (FP64 sums of
-    the reference's float terms).  S_sub is the default-method spectrum of the
-    first len(...) particles; contraction_samples are sampled rows of the batched
-    contraction (or None)."""
+def step_roofline(n, ms):
+    """The whole step against the HBM roofline of its essential traffic: every
+    particle's u and B read once (24 B/particle); the factorised form's
+    intermediate (sorted sources, run tables, moments) is not counted."""
+    nbytes = 24 * n
+    achieved = nbytes / (ms * 1e-3)
+    return {"bound": "hbm", "essential_bytes": nbytes, "achieved": achieved / 1e9, "unit": "GB/s",
+            "peak": PEAK_HBM / 1e9, "frac": achieved / PEAK_HBM, "ms": ms}
+
+
+def oracle_checks(u, b, bins, S_step, contraction_samples):
+    """The checker: the timed path's outputs against the oracle.  S_step is the
+    default-method spectrum of all the step's particles: its max relative error
+    per bin (bins above 1e-12 of the peak) against the reference's float terms
+    summed in FP64 (oracle EXACT) and against the FP64 model; contraction_samples
+    are sampled rows of the batched contraction (or None)."""
     import numpy as np
 
     from oracle import oracle as orc
 
     out = {}
-    _, ref = orc.sync_spectrum(u, b, bins, want_fp64=True)
-    live = ref > 1e-12 * ref.max()
-    out["parity_max_rel_err_subset"] = float(np.max(np.abs(S_sub[live] / ref[live] - 1)))
+    table = orc.ftable()
+    _, ex = orc.sync_spectrum(u, b, bins, mode=orc.EXACT, table=table, want_fp64=True)
+    _, f64 = orc.sync_spectrum(u, b, bins, mode=orc.FP64, table=table, want_fp64=True)
+    live = ex > 1e-12 * ex.max()
+    out["parity_max_rel_err"] = float(np.max(np.abs(S_step[live] / ex[live] - 1)))
+    out["accuracy_max_rel_err_vs_fp64"] = float(np.max(np.abs(S_step[live] / f64[live] - 1)))
+    out["reference_fp32_max_rel_err_vs_fp64"] = float(np.max(np.abs(ex[live] / f64[live] - 1)))
     if contraction_samples is not None:
         g, cb, f = contraction_samples["g"], contraction_samples["bins"], contraction_samples["f"]
         errs = {}
         for name, rows in contraction_samples["out"].items():
             worst = 0.0
             for i in range(f.shape[0]):
-                _, f64 = orc.sync_fromdist(g, f[i], True, cb, mode=orc.FP64, want_fp64=True)
-                lv = f64 > 1e-6 * f64.max()
+                fi = f[i]
+                if name.endswith("_generated"):
+                    # the generated row: x^p exp(-x / xc) / trapezoid norm, p and xc as floats
+                    gd = g.astype(np.float64)
+                    v = np.exp(np.float64(np.float32(contraction_samples["index"][i])) * np.log(gd)
+                               - gd / np.float64(np.float32(contraction_samples["cutoff"][i])))
+                    fi = (v / np.trapezoid(v, gd)).astype(np.float32)
+                _, f64 = orc.sync_fromdist(g, fi, True, cb, mode=orc.FP64, want_fp64=True)
+                lv = f64 > 1e-12 * f64.max()
                 worst = max(worst, float(np.max(np.abs(rows[i][lv] / f64[lv] - 1))))
             errs[name] = worst
         out["contraction_max_rel_err_vs_fp64_32rows"] = errs
@@ -584,6 +654,88 @@ def cpu_baseline(u, b, bins, target_cpu_seconds):
     return {"value": evals / dt, "unit": "evals/s", "cores": threads, "kind": "port",
             "sample": f"first {n} particles of the workload x {len(bins)} bins "
                       f"({evals:.3e} evals, {dt:.2f} s wall on {threads} threads)"}
+
+
+def _timed(fn, min_seconds):
+    """(calls, wall seconds) of repeated fn() over at least min_seconds."""
+    k, t = 0, time.perf_counter()
+    while True:
+        fn()
+        k += 1
+        dt = time.perf_counter() - t
+        if dt >= min_seconds:
+            return k, dt
+
+
+def cpu_baselines(u, b, bins, cells_sample, contraction_samples, seconds=2.0):
+    """The oracle's faithful restatement of the reference (per-thread float
+    accumulators, per-pair FP64 OmegaSync, log10f interpolation, the F table
+    rebuilt on every FromDist call as the reference does), on bounded samples
+    of each workload, on this host's cores."""
+    import numpy as np
+    from concurrent.futures import ThreadPoolExecutor
+
+    from oracle import oracle as orc
+
+    threads = orc.num_threads()
+    out = {}
+    # distribution form, 200 x 200 (one call builds the table, then sums 40k pairs)
+    g = orc.logspace(1, 100, 200)
+    f = orc.generator("plaw", [-2.0, 1.0, 100.0], g)
+    db = orc.logspace(0.01, 1e7, 200)
+    k, dt = _timed(lambda: orc.sync_fromdist(g, f, True, db, mode=orc.FAITHFUL, table=orc.ftable()), seconds)
+    out["fromdist_200x200"] = {"value": dt / k * 1e6, "unit": "us/call", "cores": 1, "kind": "port",
+                               "sample": f"{k} calls, each building the F table as the reference does"}
+    # energy histogram (12 B per particle read)
+    hb = orc.logspace(1e-2, 1e3, 200)
+    m = min(u.shape[1], 1 << 22)
+    k, dt = _timed(lambda: orc.energy_hist(u[:, :m], hb, True, True, mode=orc.FAITHFUL), seconds)
+    out["energy_hist"] = {"value": HIST_BYTES_PER_PARTICLE * m * k / dt / 1e9, "unit": "GB/s",
+                          "cores": threads, "kind": "port",
+                          "sample": f"{k} histograms of the first {m} particles of the workload"}
+    # weighted forward of the fit (the reference has no adjoint)
+    from ragnar_amd import workloads
+
+    m = 1 << 18
+    w = workloads.fit_weights(u[:, :m])
+    tab = orc.ftable()
+    k, dt = _timed(lambda: orc.sync_spectrum(u[:, :m], b[:, :m], bins, w=w, mode=orc.FAITHFUL, table=tab),
+                   seconds)
+    out["fit_forward"] = {"value": m * len(bins) * k / dt, "unit": "evals/s", "cores": threads, "kind": "port",
+                          "sample": f"forward only (the reference has no adjoint), first {m} particles "
+                                    "of the headline workload with the fit's weights x 256 bins"}
+    # batched cells: per-cell calls, as the reference would make them
+    if cells_sample is not None:
+        cu, cb, offs, total, ncells = cells_sample
+        cbins = orc.logspace(1e-2, 1e7, 256)
+
+        def cells():
+            for c in range(len(offs) - 1):
+                orc.sync_spectrum(cu[:, offs[c]:offs[c + 1]], cb[:, offs[c]:offs[c + 1]], cbins,
+                                  mode=orc.FAITHFUL, table=orc.ftable())
+
+        k, dt = _timed(cells, seconds)
+        n_s = int(offs[-1] - offs[0])
+        rate = n_s * 256 * k / dt
+        out["cells_4096"] = {"value": total * 256 / rate * 1e3, "unit": "ms (extrapolated)", "cores": threads,
+                             "kind": "port", "evals_per_s": rate,
+                             "sample": f"the first {len(offs) - 1} of {ncells} cells ({n_s} particles), one "
+                                       "call per cell with its table build, scaled to all particles"}
+    # batched contraction: one FromDist call per row, rows spread over the threads
+    if contraction_samples is not None:
+        cg, cbb, cf = contraction_samples["g"], contraction_samples["bins"], contraction_samples["f"]
+
+        def row(i):
+            orc.sync_fromdist(cg, cf[i % cf.shape[0]], True, cbb, mode=orc.FAITHFUL, table=orc.ftable())
+
+        with ThreadPoolExecutor(threads) as ex:
+            k, dt = _timed(lambda: list(ex.map(row, range(threads * 2))), seconds)
+        rows_per_s = threads * 2 * k / dt
+        out["contraction_65536x512x512"] = {
+            "value": 65536 / rows_per_s * 1e3, "unit": "ms (extrapolated)", "cores": threads, "kind": "port",
+            "sample": f"{threads * 2 * k} FromDist calls (512 x 512, table built per call) on {threads} "
+                      "threads, scaled to 65536 rows"}
+    return out
 
 
 def main():
@@ -627,6 +779,7 @@ def main():
     out = torch.empty(nb, device=dev)
     evals = n * nb
 
+    progress("inputs resident; timing both methods")
     # both methods, then the step the library runs by default (method "auto")
     method_ms = {}
     for m in ("direct", "binned"):
@@ -641,9 +794,8 @@ def main():
     auto = stage_profile(T, plan, U, B, out, torch, reps=20)
     auto_method = "binned" if auto["kernels"][0][0].startswith("binned") else "direct"
 
-    # the default method's output on a subset, for the checker (CPU leg)
-    sub = 1 << 14
-    S_sub = T.sync_spectrum(plan, U[:, :sub].contiguous(), B[:, :sub].contiguous()).cpu().numpy()
+    # the default method's output, for the checker (CPU leg)
+    S_step = T.sync_spectrum(plan, U, B).double().cpu().numpy()
     S_full = {m: T.sync_spectrum(plan, U, B, method=m).double().cpu().numpy() for m in ("direct", "binned")}
     live = S_full["direct"] > 1e-12 * S_full["direct"].max()
     methods_agree = float(np.max(np.abs(S_full["binned"][live] / S_full["direct"][live] - 1)))
@@ -665,12 +817,13 @@ def main():
                                "bins [1e-2, 1e7] mc^2, |u| power law p=-2.5 on [1,1e3], "
                                "B=(0,0,1)+N(0,0.3^2), unit weights",
                    "particles": n, "photon_bins": nb, "method": auto_method},
-        "roofline": roofline(auto, n, nb, plan),
+        "roofline": dict(roofline(auto, n, nb, plan), step=step_roofline(n, ms)),
         "methods_ms": method_ms,
         "methods_max_rel_diff": methods_agree,
         "stages_ms": {name: t for name, t in auto["kernels"]},
     }
 
+    progress(f"headline step {ms:.4f} ms")
     extras = {}
     if not args.no_extras:
         # forward + adjoint (differentiable fit step at the same size)
@@ -693,11 +846,13 @@ def main():
         extras["fwd_adj"] = {"ms": ms_fa, "evals_per_s": evals / (ms_fa * 1e-3), "ms_fwd_weighted": ms_fw,
                              "ratio_to_fwd": ms_fa / ms_fw, "ratio_to_unweighted_fwd": ms_fa / ms,
                              "adjoint_ms_by_method": adj_ms}
+        progress("fwd+adjoint done; fit at 2^26")
         # differentiable fit at 2^26 particles (fwd + adjoint of the reweighting loss)
         try:
             extras["fit_2p26"] = bench_fit(T, workloads, torch, table, bins)
         except Exception as exc:  # reported, the headline line still prints
             extras["fit_2p26"] = {"error": repr(exc)}
+        progress("fit done; histogram, FromDist, IC")
         # energy histogram (HBM bound: 12 B per particle)
         hbins = logspace(1e-2, 1e3, 200)
         hplan = T.HistPlan(hbins, True)
@@ -719,6 +874,7 @@ def main():
                                                      logspace(1e3, 1e7, 200))]
         ms_i = event_time(lambda: T.ic_spectrum(ti[0], ti[1], True, ti[2], ti[3], ti[4]), 50, 5, torch)
         extras["ic_200x200x200"] = {"us": ms_i * 1e3, "triples_per_s": 200 ** 3 / (ms_i * 1e-3)}
+        progress("cells and contraction")
         for name, fn in (("cells_4096", lambda: bench_cells(T, workloads, torch, table)),
                          ("contraction_65536x512x512",
                           lambda: bench_contraction(T, workloads, torch, table))):
@@ -730,15 +886,22 @@ def main():
     c = extras.get("contraction_65536x512x512")
     if isinstance(c, tuple):
         extras["contraction_65536x512x512"], samples = c
+    cells_sample = None
+    c = extras.get("cells_4096")
+    if isinstance(c, tuple):
+        extras["cells_4096"], cells_sample = c
     res["extras"] = extras
 
     # CPU leg: the oracle times the reference algorithm and checks the outputs
     if not args.no_cpu:
-        checks = oracle_checks(u_h[:, :sub], b_h[:, :sub], bins, S_sub, samples)
-        res["parity_max_rel_err_subset"] = checks["parity_max_rel_err_subset"]
+        progress("CPU leg: oracle checks and baselines")
+        checks = oracle_checks(u_h, b_h, bins, S_step, samples)
+        for key in ("parity_max_rel_err", "accuracy_max_rel_err_vs_fp64", "reference_fp32_max_rel_err_vs_fp64"):
+            res[key] = checks[key]
         for name, err in checks.get("contraction_max_rel_err_vs_fp64_32rows", {}).items():
             extras["contraction_65536x512x512"][name]["max_rel_err_vs_fp64_32rows"] = err
         res["cpu_baseline"] = cpu_baseline(u_h, b_h, bins, args.cpu_seconds)
+        extras["cpu_baselines"] = cpu_baselines(u_h, b_h, bins, cells_sample, samples)
     print(json.dumps(res))
 
 
