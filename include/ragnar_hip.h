@@ -229,6 +229,23 @@ int rgn_hist_plan_destroy(rgn_hist_plan* plan);
 int rgn_energy_hist(const rgn_hist_plan* plan, const float* const d_u[3], size_t n,
                     int fourvel, float* d_hist, uint64_t* d_counts, rgn_stream_t stream);
 
+/* rgn_energy_hist without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * per-block partials live in a fixed 8 MiB device array of the library (each
+ * device holds its own copy until the process ends) instead of the library's
+ * scratch; the call makes no hipMalloc/hipFree and no allocating library
+ * call.  The grid is cut so that bins x blocks x 12 bytes fits the array
+ * (the full grid of 2048 blocks up to 341 bins; fewer blocks beyond), so any
+ * plan and particle count are accepted; counts equal rgn_energy_hist's bit
+ * for bit and sums agree to FP64 rounding.  Every call on a device shares its
+ * array, which keeps the last call's contents (each call writes every entry
+ * it reads, so nothing needs clearing): calls that overlap on one device, from
+ * different streams or threads, race, and the caller orders them.  The plan
+ * is created beforehand (rgn_hist_plan_create allocates). */
+int rgn_energy_hist_noalloc(const rgn_hist_plan* plan, const float* const d_u[3], size_t n,
+                            int fourvel, float* d_hist, uint64_t* d_counts, rgn_stream_t stream);
+
 /* ---- inverse-Compton spectrum ------------------------------------------------ */
 /* ICSpectrum (src/physics/ic.cpp:15-46, per-triple kernel src/physics/ic.hpp:28-86):
  * E_ic^2 f(E_ic) at the neic energies d_eic (mc^2 units) of particles with

@@ -61,6 +61,7 @@ _SIGNATURES = {
     "rgn_hist_plan_destroy": (C.c_int, [vp]),
     "rgn_hist_plan_edges": (C.c_int, [vp, vp]),
     "rgn_energy_hist": (C.c_int, [vp, fptr3, sz, C.c_int, vp, vp, vp]),
+    "rgn_energy_hist_noalloc": (C.c_int, [vp, fptr3, sz, C.c_int, vp, vp, vp]),
     "rgn_malloc": (C.c_int, [C.POINTER(vp), sz]),
     "rgn_free": (C.c_int, [vp]),
     "rgn_memcpy_h2d": (C.c_int, [vp, vp, sz]),

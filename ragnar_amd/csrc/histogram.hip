@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <vector>
 
 #include "rgn_core.hpp"
@@ -300,6 +301,35 @@ void launch_hist(const rgn_hist_plan& plan, const float* const d_u[3], size_t n,
 }  // namespace
 }  // namespace rgn
 
+// Per-block partials of rgn_energy_hist_noalloc: a fixed device array (each
+// device holds its own copy until the process ends), used instead of the
+// library's allocated scratch.  Host code reaches it only through the address
+// hipGetSymbolAddress returns on each device (noalloc_partials below).
+constexpr size_t kHistNoAllocBytes = size_t{8} << 20;
+static __device__ __align__(256) unsigned char scratch[kHistNoAllocBytes];
+
+namespace {
+std::mutex g_noalloc_mutex;
+constexpr int kMaxDevices = 64;
+void* g_noalloc_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up on the first
+// call on that device and kept.
+void* noalloc_partials() {
+  int dev = 0;
+  RGN_HIP(hipGetDevice(&dev));
+  RGN_REQUIRE(dev >= 0 && dev < kMaxDevices, RGN_ERR_UNSUPPORTED, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_noalloc_mutex);
+  if (!g_noalloc_ptr[dev]) {
+    void* p = nullptr;
+    RGN_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(::scratch)));
+    RGN_REQUIRE(p != nullptr, RGN_ERR_HIP, "no device address for the histogram's fixed scratch");
+    g_noalloc_ptr[dev] = p;
+  }
+  return g_noalloc_ptr[dev];
+}
+}  // namespace
+
 using namespace rgn;
 
 extern "C" {
@@ -360,6 +390,22 @@ int rgn_energy_hist(const rgn_hist_plan* plan, const float* const d_u[3], size_t
     const HistShape sh = hist_shape(*plan, d_u, n);
     double* partials = static_cast<double*>(rgn::scratch(hist_partial_bytes(plan->nb, sh.grid), 2));
     launch_hist(*plan, d_u, n, fourvel, d_hist, d_counts, sh, partials, st);
+  });
+}
+
+int rgn_energy_hist_noalloc(const rgn_hist_plan* plan, const float* const d_u[3], size_t n,
+                            int fourvel, float* d_hist, uint64_t* d_counts, rgn_stream_t stream) {
+  return guarded([&] {
+    RGN_REQUIRE(plan != nullptr, RGN_ERR_INVALID, "null plan");
+    RGN_REQUIRE(d_u && d_u[0] && d_u[1] && d_u[2] && d_hist, RGN_ERR_INVALID, "null argument");
+    HistShape sh = hist_shape(*plan, d_u, n);
+    // the grid is cut to what the fixed array holds: bins x blocks x 12 bytes
+    const size_t per_block = hist_partial_bytes(plan->nb, 1);
+    RGN_REQUIRE(per_block <= kHistNoAllocBytes, RGN_ERR_UNSUPPORTED,
+                "energy bins beyond the fixed scratch of rgn_energy_hist_noalloc");
+    sh.grid = static_cast<int>(std::min<size_t>(sh.grid, kHistNoAllocBytes / per_block));
+    double* partials = static_cast<double*>(noalloc_partials());
+    launch_hist(*plan, d_u, n, fourvel, d_hist, d_counts, sh, partials, as_stream(stream));
   });
 }
 
