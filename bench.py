@@ -135,8 +135,9 @@ PEAK_MFMA_F16 = 2.5e15         # dense
 
 def profile_compute_run(args):
     """Single dispatches of the compute-bound kernels for rocprofv3: the
-    pair-by-pair forward at 2^24 x 256 (FP32 VALU), the split-FP16 batched
-    contraction at 65536 x 512 x 512, three and one products (MFMA), and the
+    pair-by-pair forward at 2^24 x 256 (FP32 VALU), the batched contraction at
+    65536 x 512 x 512 (MFMA): split FP16 with three and one products, split
+    BF16 with six, and the rows generated in the kernel, and the
     inverse-Compton spectrum at 1000 x 1000 x 1000 triples (FP32/FP64 VALU)."""
     import torch
 
@@ -150,9 +151,11 @@ def profile_compute_run(args):
     plan = T.SyncPlan(T.default_table(), logspace(1e-2, 1e7, nb))
     S = torch.empty(nb, device="cuda")
     nd, ng, nbc = 65536, 512, 512
-    f, _, _ = workloads.distributions(nd, ng)
+    f, pidx, gcut = workloads.distributions(nd, ng)
     G = torch.from_numpy(logspace(1, 1e6, ng)).cuda()
     Fm = torch.from_numpy(f).cuda()
+    Pi = torch.from_numpy(pidx.astype("float32")).cuda()
+    Gc = torch.from_numpy(gcut.astype("float32")).cuda()
     cplan = T.SyncPlan(T.default_table(), logspace(1e-2, 1e7, nbc))
     out = torch.empty(nd, nbc, device="cuda")
     reps = args.profile_reps
@@ -163,10 +166,14 @@ def profile_compute_run(args):
     print(f"profile: forward (direct) x{reps}", flush=True)
     for _ in range(reps):
         T.sync_spectrum(plan, U, B, out=S, method="direct")
-    for prec, name in ((1, "f16x3"), (2, "f16x1")):
+    for prec, name in ((1, "f16x3"), (2, "f16x1"), (3, "bf16x6")):
         print(f"profile: contraction {name} x{reps}", flush=True)
         for _ in range(reps):
             T.sync_fromdist_batched(cplan, G, Fm, True, prec, out=out)
+    for prec, name in ((1, "f16x3"), (3, "bf16x6")):
+        print(f"profile: contraction {name}, rows generated in the kernel x{reps}", flush=True)
+        for _ in range(reps):
+            T.sync_fromdist_batched_plaw(cplan, G, Pi, Gc, True, prec, out=out)
     gi = logspace(1e3, 1e7, 1000)
     si = logspace(1e-11, 1e-7, 1000)
     ic = [torch.from_numpy(a).cuda() for a in (gi, generator("plaw", [-1.5, 1e3, 1e7], gi), si,
@@ -226,13 +233,15 @@ def summarize_compute(trace_dir, pmc_dir):
         return statistics.mean(v[1:] if len(v) > 1 else v)
 
     rows = ["| kernel | dispatches | avg duration (us) | FP32 VALU TFLOP/s | FP64 VALU TFLOP/s | "
-            "F16 MFMA TFLOP/s | VALUBusy % | MfmaUtil % |", "|---|---|---|---|---|---|---|---|"]
+            "F16 MFMA TFLOP/s (counter) | matrix TFLOP/s (products) | VALUBusy % | MfmaUtil % |",
+            "|---|---|---|---|---|---|---|---|---|"]
+    gemm = 2.0 * 65536 * 512 * 512  # one product pass of the profiled contraction
     report = {}
     for k, d in sorted(dur.items(), key=lambda kv: -statistics.mean(kv[1])):
         full = demangle(k)
         short = _short_name(full)
-        if short not in ("forward_kernel", "combine_kernel", "mfma_contract", "pack_kernel_matrix",
-                         "ic_kernel", "ic_combine"):
+        if short not in ("forward_kernel", "combine_kernel", "mfma_contract", "mfma_contract2", "mfma_contract_bf16",
+                         "pack_kernel_matrix", "pack_kernel_matrix_bf16", "plaw_rows", "ic_kernel", "ic_combine"):
             continue
         c = vals.get(k, {})
         t = statistics.mean(d[1:] if len(d) > 1 else d)
@@ -249,15 +258,25 @@ def summarize_compute(trace_dir, pmc_dir):
         base = full.replace("(anonymous namespace)::", "").split("(")[0].replace("void ", "").strip()
         tmpl = short + (base[base.index("<"):] if "<" in base else "")
         tmpl = tmpl.replace("rgn::", "")
-        rows.append(f"| `{tmpl}` | {len(d)} | {t * 1e6:.1f} | {f32:.1f} | {f64:.1f} | {mf:.0f} | {vb:.0f} | {mu:.0f} |")
+        # products per element of the contraction kernels (template argument
+        # NPROD of the split-FP16 kernels; six for split BF16)
+        nprod = 6 if short == "mfma_contract_bf16" else (
+            int(base[base.index("<") + 1:].split(",")[0]) if short.startswith("mfma_contract") and "<" in base else 0)
+        mx = nprod * gemm / t / 1e12 if nprod else float("nan")
+        rows.append(f"| `{tmpl}` | {len(d)} | {t * 1e6:.1f} | {f32:.1f} | {f64:.1f} | {mf:.0f} | {mx:.0f} | {vb:.0f} | "
+                    f"{mu:.0f} |")
         report[tmpl] = {"us": t * 1e6, "fp32_tflops": f32, "fp64_tflops": f64, "mfma_f16_tflops": mf,
-                        "valu_busy_pct": vb, "mfma_util_pct": mu}
+                        "matrix_tflops": mx, "valu_busy_pct": vb, "mfma_util_pct": mu}
     md = ["# VALU and MFMA rates of the compute-bound kernels (MI355X)", "",
           "Made by `python bench.py --summarize-compute <trace> <pmc>` from two rocprofv3 runs of",
           "`bench.py --profile-compute` (commands in profiles/README.md): the pair-by-pair",
-          "forward at 2^24 particles x 256 bins, the split-FP16 contraction at 65536 x 512 x 512",
-          "(`mfma_contract<3, true>`: three products; `<1, true>`: one) and the inverse-Compton",
-          "spectrum at 1000 gamma x 1000 soft-photon x 1000 output bins (`ic_kernel`).", "",
+          "forward at 2^24 particles x 256 bins, the batched contraction at 65536 x 512 x 512",
+          "(split FP16 `mfma_contract<3, true>`: three products, `<1, true>`: one; split BF16",
+          "`mfma_contract_bf16<true, false>`: six products; `<false, true>` and",
+          "`mfma_contract2<3, false, true>`: rows generated in the kernel) and the inverse-Compton",
+          "spectrum at 1000 gamma x 1000 soft-photon x 1000 output bins (`ic_kernel`).  The",
+          "matrix rate counts every product pass (2 x 65536 x 512 x 512 flop each) over the traced",
+          "duration; the counter rate is F16 only.", "",
           f"Peaks: FP32 VALU {PEAK_FP32_NONPACKED / 1e12:.1f} TFLOP/s without packed math "
           f"({PEAK_FP32 / 1e12:.1f} with it; this build compiles with packed FP32 off), "
           f"FP64 VALU {PEAK_FP64_VALU / 1e12:.1f}, dense F16 MFMA {PEAK_MFMA_F16 / 1e12:.0f}.",

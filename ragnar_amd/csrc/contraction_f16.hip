@@ -55,9 +55,20 @@ __device__ __forceinline__ void split2(float x, int s, _Float16& hi, _Float16& l
 
 // K[g][e] (ng x nb, FP32) -> Bh/Bl[(kb * nbp + c) * 64 + i] (g = 64 kb + i), and
 // the exponent of each (kb, c).  One thread per (column, g block).
+// Packed index of (g block kb, column c, i = g - 64 kb).  Block-major
+// (ksmajor = 0): [kb][c][64], a tile's block one contiguous range (copied
+// through LDS).  Step-major (ksmajor = 1): [kb][i / 16][c][16], so the 16 g of
+// one MFMA step of one column are 32 contiguous bytes and a wave's fragment
+// load (32 columns, two halves of 8 g) reads 1 KB contiguously.
+__device__ __forceinline__ size_t packed_index(int kb, int c, int i, int nbp, int ksmajor) {
+  return ksmajor ? ((static_cast<size_t>(kb) * (kKB / 16) + i / 16) * nbp + c) * 16 + i % 16
+                 : (static_cast<size_t>(kb) * nbp + c) * kKB + i;
+}
+
 __global__ void __launch_bounds__(256) pack_kernel_matrix(const float* __restrict__ K, int ng, int nb, int nbp,
                                                           _Float16* __restrict__ Bh, _Float16* __restrict__ Bl,
-                                                          int* __restrict__ sB, float* __restrict__ rB) {
+                                                          int* __restrict__ sB, float* __restrict__ rB,
+                                                          int ksmajor) {
   const int c = blockIdx.x * 256 + threadIdx.x;
   const int kb = blockIdx.y;
   if (c >= nbp) return;
@@ -70,13 +81,13 @@ __global__ void __launch_bounds__(256) pack_kernel_matrix(const float* __restric
     m = fmaxf(m, fabsf(x[i]));
   }
   const int s = block_exp(m);
-  const size_t base = (static_cast<size_t>(kb) * nbp + c) * kKB;
 #pragma unroll
   for (int i = 0; i < kKB; ++i) {
     _Float16 hi, lo;
     split2(x[i], s, hi, lo);
-    Bh[base + i] = hi;
-    Bl[base + i] = lo;
+    const size_t at = packed_index(kb, c, i, nbp, ksmajor);
+    Bh[at] = hi;
+    Bl[at] = lo;
   }
   sB[static_cast<size_t>(kb) * nbp + c] = s;
   rB[static_cast<size_t>(kb) * nbp + c] = ldexpf(1.0f, -s);
@@ -267,8 +278,9 @@ struct GenRows {
 
 __device__ __forceinline__ float gen_value(double p, double c, double o, double L, double x) {
   const double t = fma(p, L, fma(-c, x, o));
-  const double n = floor(t);
-  if (n < -160.0) return 0.0f;  // below the float range
+  // no branch: elements interleave; exponents below -1000 are clamped, and
+  // ldexpf flushes anything below the float range to zero
+  const double n = fmax(floor(t), -1000.0);
   return ldexpf(__builtin_amdgcn_exp2f(static_cast<float>(t - n)), static_cast<int>(n));
 }
 
@@ -295,17 +307,26 @@ __global__ void __launch_bounds__(kThreads2, 2) mfma_contract2(const float* __re
   const int d = d0 + ar;
   float ra[16];
   double gp = 0.0, gc = 0.0, go = 0.0;
-  if (GEN && d < nd) {
-    gp = gen.rows[3 * static_cast<size_t>(d)];
-    gc = gen.rows[3 * static_cast<size_t>(d) + 1];
-    go = gen.rows[3 * static_cast<size_t>(d) + 2];
+  // generated rows: the grid terms L_g, x_g in LDS (read by every row)
+  extern __shared__ __align__(16) double gterms[];  // GEN: [2][ng]
+  if (GEN) {
+    for (int i = threadIdx.x; i < ng; i += blockDim.x) {
+      gterms[i] = gen.L[i];
+      gterms[ng + i] = gen.x[i];
+    }
+    __syncthreads();
+    if (d < nd) {
+      gp = gen.rows[3 * static_cast<size_t>(d)];
+      gc = gen.rows[3 * static_cast<size_t>(d) + 1];
+      go = gen.rows[3 * static_cast<size_t>(d) + 2];
+    }
   }
   auto load_a = [&](int kb) {
     const int g0 = kb * kKB + aq * 16;
     if (GEN) {
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
-        ra[i] = (d < nd && g0 + i < ng) ? gen_value(gp, gc, go, gen.L[g0 + i], gen.x[g0 + i]) : 0.0f;
+        ra[i] = (d < nd && g0 + i < ng) ? gen_value(gp, gc, go, gterms[g0 + i], gterms[ng + g0 + i]) : 0.0f;
       }
     } else if (VEC) {
 #pragma unroll
@@ -344,10 +365,11 @@ __global__ void __launch_bounds__(kThreads2, 2) mfma_contract2(const float* __re
       if (NPROD == 3) *reinterpret_cast<half8*>(&Al[buf][ar][aq * 16 + k * 8]) = l;
     }
   };
-  // K fragments of this wave: column wc + 32 j + lane % 32, g = 16 ks + 8 (lane / 32) .. + 8
+  // K fragments of this wave (step-major layout): column wc + 32 j + lane % 32,
+  // g = 16 ks + 8 (lane / 32) .. + 8
   const int kofs = (lane >> 5) * 8;
   auto b_addr = [&](int kb, int j, int ks) {
-    return (static_cast<size_t>(kb) * nbp + c0 + wc + j * 32 + (lane & 31)) * kKB + ks * 16 + kofs;
+    return packed_index(kb, c0 + wc + j * 32 + (lane & 31), ks * 16 + kofs, nbp, 1);
   };
   float16v acc[2][2];
 #pragma unroll
@@ -479,22 +501,22 @@ __device__ __forceinline__ void split3(float x, __bf16& h, __bf16& m, __bf16& l)
   l = static_cast<__bf16>(r1 - static_cast<float>(m));  // exact difference
 }
 
-// K[g][e] -> B{h,m,l}[(kb * nbp + c) * 64 + i] (g = 64 kb + i).
+// K[g][e] -> B{h,m,l} in the step-major packed layout.
 __global__ void __launch_bounds__(256) pack_kernel_matrix_bf16(const float* __restrict__ K, int ng, int nb, int nbp,
                                                                __bf16* __restrict__ Bh, __bf16* __restrict__ Bm,
                                                                __bf16* __restrict__ Bl) {
   const int c = blockIdx.x * 256 + threadIdx.x;
   const int kb = blockIdx.y;
   if (c >= nbp) return;
-  const size_t base = (static_cast<size_t>(kb) * nbp + c) * kKB;
   for (int i = 0; i < kKB; ++i) {
     const int g = kb * kKB + i;
     const float x = (c < nb && g < ng) ? K[static_cast<size_t>(g) * nb + c] : 0.0f;
     __bf16 h, m, l;
     split3(x, h, m, l);
-    Bh[base + i] = h;
-    Bm[base + i] = m;
-    Bl[base + i] = l;
+    const size_t at = packed_index(kb, c, i, nbp, 1);
+    Bh[at] = h;
+    Bm[at] = m;
+    Bl[at] = l;
   }
 }
 
@@ -517,17 +539,26 @@ __global__ void __launch_bounds__(kThreads2, 2) mfma_contract_bf16(const float* 
   const int d = d0 + ar;
   float ra[16];
   double gp = 0.0, gc = 0.0, go = 0.0;
-  if (GEN && d < nd) {
-    gp = gen.rows[3 * static_cast<size_t>(d)];
-    gc = gen.rows[3 * static_cast<size_t>(d) + 1];
-    go = gen.rows[3 * static_cast<size_t>(d) + 2];
+  // generated rows: the grid terms L_g, x_g in LDS (read by every row)
+  extern __shared__ __align__(16) double gterms[];  // GEN: [2][ng]
+  if (GEN) {
+    for (int i = threadIdx.x; i < ng; i += blockDim.x) {
+      gterms[i] = gen.L[i];
+      gterms[ng + i] = gen.x[i];
+    }
+    __syncthreads();
+    if (d < nd) {
+      gp = gen.rows[3 * static_cast<size_t>(d)];
+      gc = gen.rows[3 * static_cast<size_t>(d) + 1];
+      go = gen.rows[3 * static_cast<size_t>(d) + 2];
+    }
   }
   auto load_a = [&](int kb) {
     const int g0 = kb * kKB + aq * 16;
     if (GEN) {
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
-        ra[i] = (d < nd && g0 + i < ng) ? gen_value(gp, gc, go, gen.L[g0 + i], gen.x[g0 + i]) : 0.0f;
+        ra[i] = (d < nd && g0 + i < ng) ? gen_value(gp, gc, go, gterms[g0 + i], gterms[ng + g0 + i]) : 0.0f;
       }
     } else if (VEC) {
 #pragma unroll
@@ -563,7 +594,7 @@ __global__ void __launch_bounds__(kThreads2, 2) mfma_contract_bf16(const float* 
   };
   const int kofs = (lane >> 5) * 8;
   auto b_addr = [&](int kb, int j, int ks) {
-    return (static_cast<size_t>(kb) * nbp + c0 + wc + j * 32 + (lane & 31)) * kKB + ks * 16 + kofs;
+    return packed_index(kb, c0 + wc + j * 32 + (lane & 31), ks * 16 + kofs, nbp, 1);
   };
   float16v acc[2][2];
 #pragma unroll
@@ -577,29 +608,28 @@ __global__ void __launch_bounds__(kThreads2, 2) mfma_contract_bf16(const float* 
   load_a(0);
   split_a(0);
   if (nkb > 1) load_a(1);
-  bf16x8 bh[2], bm[2], bl[2];
+  // K fragments two 16-g steps ahead: b? holds step s, n1? step s + 1
+  bf16x8 bh[2], bm[2], bl[2], n1h[2], n1m[2], n1l[2];
+  auto load_b = [&](int step, bf16x8* h, bf16x8* m, bf16x8* l) {
+    const int kb = step / (kKB / 16), ks = step % (kKB / 16);
+    if (kb < nkb) {
 #pragma unroll
-  for (int j = 0; j < 2; ++j) {
-    bh[j] = *reinterpret_cast<const bf16x8*>(Bh + b_addr(0, j, 0));
-    bm[j] = *reinterpret_cast<const bf16x8*>(Bm + b_addr(0, j, 0));
-    bl[j] = *reinterpret_cast<const bf16x8*>(Bl + b_addr(0, j, 0));
-  }
+      for (int j = 0; j < 2; ++j) {
+        h[j] = *reinterpret_cast<const bf16x8*>(Bh + b_addr(kb, j, ks));
+        m[j] = *reinterpret_cast<const bf16x8*>(Bm + b_addr(kb, j, ks));
+        l[j] = *reinterpret_cast<const bf16x8*>(Bl + b_addr(kb, j, ks));
+      }
+    }
+  };
+  load_b(0, bh, bm, bl);
+  load_b(1, n1h, n1m, n1l);
   __syncthreads();
   for (int kb = 0; kb < nkb; ++kb) {
     const int cur = kb & 1;
 #pragma unroll
     for (int ks = 0; ks < kKB / 16; ++ks) {
       bf16x8 nbh[2], nbm[2], nbl[2];
-      const int nks = (ks + 1) % (kKB / 16);
-      const int nkbb = ks + 1 < kKB / 16 ? kb : kb + 1;
-      if (nkbb < nkb) {
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          nbh[j] = *reinterpret_cast<const bf16x8*>(Bh + b_addr(nkbb, j, nks));
-          nbm[j] = *reinterpret_cast<const bf16x8*>(Bm + b_addr(nkbb, j, nks));
-          nbl[j] = *reinterpret_cast<const bf16x8*>(Bl + b_addr(nkbb, j, nks));
-        }
-      }
+      load_b(kb * (kKB / 16) + ks + 2, nbh, nbm, nbl);
       const int kk = ks * 16 + kofs;
       bf16x8 ah[2], am[2], al[2];
 #pragma unroll
@@ -621,13 +651,14 @@ __global__ void __launch_bounds__(kThreads2, 2) mfma_contract_bf16(const float* 
           acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
         }
       }
-      if (nkbb < nkb) {
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          bh[j] = nbh[j];
-          bm[j] = nbm[j];
-          bl[j] = nbl[j];
-        }
+      for (int j = 0; j < 2; ++j) {
+        bh[j] = n1h[j];
+        bm[j] = n1m[j];
+        bl[j] = n1l[j];
+        n1h[j] = nbh[j];
+        n1m[j] = nbm[j];
+        n1l[j] = nbl[j];
       }
       if (ks == 1 && kb + 1 < nkb) {
         split_a(cur ^ 1);
@@ -709,13 +740,15 @@ void contraction_f16(const SyncPlan& plan, const float* d_k, int ng, const float
   _Float16* Bl = Bh + packed;
   int* sB = reinterpret_cast<int*>(Bl + packed);
   float* rB = reinterpret_cast<float*>(sB + static_cast<size_t>(nkb) * nbp);
-  pack_kernel_matrix<<<dim3(nbp / 256 + (nbp % 256 != 0), nkb), 256, 0, st>>>(d_k, ng, nb, nbp, Bh, Bl, sB, rB);
+  const bool form1 = env_int("RAGNAR_CONTRACT_FORM", 2) == 1;
+  pack_kernel_matrix<<<dim3(nbp / 256 + (nbp % 256 != 0), nkb), 256, 0, st>>>(d_k, ng, nb, nbp, Bh, Bl, sB, rB,
+                                                                              form1 ? 0 : 1);
   RGN_HIP(hipGetLastError());
   stage_mark(st, "pack_kernel_matrix");
   // 16-byte loads of f when every row starts 16-byte aligned
   const bool vec = ng % 4 == 0 && reinterpret_cast<uintptr_t>(d_f) % 16 == 0;
   const int nd_i = static_cast<int>(nd);
-  if (env_int("RAGNAR_CONTRACT_FORM", 1) == 1) {
+  if (form1) {
     const dim3 grid(static_cast<unsigned>(ceil_div(nd, kTM)), static_cast<unsigned>(nbp / kTN));
 #define RGN_CONTRACT(P_, V_) \
   mfma_contract<P_, V_><<<grid, kThreadsC, 0, st>>>(d_f, nd_i, ng, Bh, Bl, sB, nb, nbp, nkb, d_out)
@@ -773,20 +806,21 @@ void contraction_f16_plaw(const SyncPlan& plan, const float* d_k, int ng, const 
   _Float16* Bl = Bh + packed;
   int* sB = reinterpret_cast<int*>(Bl + packed);
   float* rB = reinterpret_cast<float*>(sB + static_cast<size_t>(nkb) * nbp);
-  pack_kernel_matrix<<<dim3(nbp / 256 + (nbp % 256 != 0), nkb), 256, 0, st>>>(d_k, ng, nb, nbp, Bh, Bl, sB, rB);
+  pack_kernel_matrix<<<dim3(nbp / 256 + (nbp % 256 != 0), nkb), 256, 0, st>>>(d_k, ng, nb, nbp, Bh, Bl, sB, rB, 1);
   RGN_HIP(hipGetLastError());
   stage_mark(st, "pack_kernel_matrix");
   const GenRows gen = prepare_gen_rows(d_gbeta, ng, d_index, d_cutoff, nd, st);
+  const size_t gsmem = 2 * sizeof(double) * static_cast<size_t>(ng);
   if (nd == 0) return;
   const int ntn = nbp / kTN;
   const unsigned grid = static_cast<unsigned>(ceil_div(nd, kTM2) * ntn);
   const int nd_i = static_cast<int>(nd);
   if (nprod == 3) {
-    mfma_contract2<3, false, true><<<grid, kThreads2, 0, st>>>(nullptr, nd_i, ng, Bh, Bl, rB, nb, nbp, nkb, ntn,
-                                                               d_out, gen);
+    mfma_contract2<3, false, true><<<grid, kThreads2, gsmem, st>>>(nullptr, nd_i, ng, Bh, Bl, rB, nb, nbp, nkb,
+                                                                   ntn, d_out, gen);
   } else {
-    mfma_contract2<1, false, true><<<grid, kThreads2, 0, st>>>(nullptr, nd_i, ng, Bh, Bl, rB, nb, nbp, nkb, ntn,
-                                                               d_out, gen);
+    mfma_contract2<1, false, true><<<grid, kThreads2, gsmem, st>>>(nullptr, nd_i, ng, Bh, Bl, rB, nb, nbp, nkb,
+                                                                   ntn, d_out, gen);
   }
   RGN_HIP(hipGetLastError());
   stage_mark(st, "mfma_contract");
@@ -816,8 +850,8 @@ void contraction_bf16(const SyncPlan& plan, const float* d_k, int ng, const floa
   const unsigned grid = static_cast<unsigned>(ceil_div(nd, kTM2) * ntn);
   const int nd_i = static_cast<int>(nd);
   if (generated) {
-    mfma_contract_bf16<false, true><<<grid, kThreads2, 0, st>>>(nullptr, nd_i, ng, Bh, Bm, Bl, nb, nbp, nkb, ntn,
-                                                                d_out, gen);
+    mfma_contract_bf16<false, true><<<grid, kThreads2, 2 * sizeof(double) * static_cast<size_t>(ng), st>>>(
+      nullptr, nd_i, ng, Bh, Bm, Bl, nb, nbp, nkb, ntn, d_out, gen);
   } else if (ng % 4 == 0 && reinterpret_cast<uintptr_t>(d_f) % 16 == 0) {
     mfma_contract_bf16<true, false><<<grid, kThreads2, 0, st>>>(d_f, nd_i, ng, Bh, Bm, Bl, nb, nbp, nkb, ntn, d_out,
                                                                 gen);
