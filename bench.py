@@ -64,8 +64,8 @@ def parse():
     ap.add_argument("--no-extras", action="store_true", help="only the headline step")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU work (thread-seconds) of the baseline sample")
-    ap.add_argument("--method", choices=("auto", "direct", "binned"), default="auto",
-                    help="emission method of --profile, and the name of --summarize's output")
+    ap.add_argument("--method", choices=("auto", "direct", "binned", "compute"), default="auto",
+                    help="emission method of --profile; with --summarize also \"compute\", the --profile-compute runs")
     ap.add_argument("--summarize", nargs=3, metavar=("TRACE_DIR", "FETCH_DIR", "WRITE_DIR"),
                     help="summarise three rocprofv3 --output-format csv runs of --profile "
                          "(--kernel-trace --stats, --pmc FETCH_SIZE, --pmc WRITE_SIZE) into "
@@ -184,6 +184,11 @@ def profile_compute_run(args):
     print(f"profile: ic spectrum 1000^3 x{reps}", flush=True)
     for _ in range(reps):
         T.ic_spectrum(ic[0], ic[1], True, ic[2], ic[3], ic[4])
+    print("profile: 1 GiB copy (calibration)", flush=True)
+    del U, B, Fm
+    src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+    src.fill_(1.0)
+    torch.empty_like(src).copy_(src)
     torch.cuda.synchronize()
     print("profile: done", flush=True)
 
@@ -195,28 +200,6 @@ def summarize_compute(trace_dir, pmc_dir):
     dispatch order of the trace) over the kernel's average traced duration."""
     import csv
     import statistics
-
-    def demangle(name):
-        """Kernel name and integer / bool template arguments of a mangled
-        _ZN...E name (what these kernels need; c++filt here cannot parse the
-        _Float16 parameter types), else the name unchanged."""
-        import re
-
-        if not name.startswith("_ZN"):
-            return name
-        i, parts = 3, []
-        while i < len(name) and name[i].isdigit():
-            m = re.match(r"\d+", name[i:])
-            n = int(m.group())
-            i += len(m.group())
-            parts.append(name[i:i + n])
-            i += n
-        args = []
-        if i < len(name) and name[i] == "I":
-            for kind, val in re.findall(r"L([ib])(\d+)E", name[i:name.find("EE", i) + 1]):
-                args.append(("true" if val == "1" else "false") if kind == "b" else val)
-        base = "::".join(p for p in parts if not p.startswith("_GLOBAL__N"))
-        return "void " + base + (f"<{', '.join(args)}>" if args else "") + "()"
 
     trace = list(csv.DictReader(open(Path(trace_dir) / "run_kernel_trace.csv")))
     dur = {}
@@ -238,7 +221,7 @@ def summarize_compute(trace_dir, pmc_dir):
     gemm = 2.0 * 65536 * 512 * 512  # one product pass of the profiled contraction
     report = {}
     for k, d in sorted(dur.items(), key=lambda kv: -statistics.mean(kv[1])):
-        full = demangle(k)
+        full = _demangle(k)
         short = _short_name(full)
         if short not in ("forward_kernel", "combine_kernel", "mfma_contract", "mfma_contract2", "mfma_contract_bf16",
                          "pack_kernel_matrix", "pack_kernel_matrix_bf16", "plaw_rows", "ic_kernel", "ic_combine"):
@@ -271,9 +254,10 @@ def summarize_compute(trace_dir, pmc_dir):
           "Made by `python bench.py --summarize-compute <trace> <pmc>` from two rocprofv3 runs of",
           "`bench.py --profile-compute` (commands in profiles/README.md): the pair-by-pair",
           "forward at 2^24 particles x 256 bins, the batched contraction at 65536 x 512 x 512",
-          "(split FP16 `mfma_contract<3, true>`: three products, `<1, true>`: one; split BF16",
-          "`mfma_contract_bf16<true, false>`: six products; `<false, true>` and",
-          "`mfma_contract2<3, false, true>`: rows generated in the kernel) and the inverse-Compton",
+          "(split FP16 `mfma_contract2<3, 2, false>`: three products, f read as whole 64-g blocks,",
+          "`<1, 2, false>`: one product; split BF16 `mfma_contract_bf16<2, false>`: six products;",
+          "`mfma_contract2<3, 0, true>` and `mfma_contract_bf16<0, true>`: rows generated in the",
+          "kernel) and the inverse-Compton",
           "spectrum at 1000 gamma x 1000 soft-photon x 1000 output bins (`ic_kernel`).  The",
           "matrix rate counts every product pass (2 x 65536 x 512 x 512 flop each) over the traced",
           "duration; the counter rate is F16 only.", "",
@@ -290,17 +274,47 @@ def summarize_compute(trace_dir, pmc_dir):
     return report
 
 
+def _demangle(name):
+    """Kernel name and integer / bool template arguments of a mangled
+    _ZN...E name (what these kernels need; c++filt here cannot parse the
+    _Float16 parameter types), else the name unchanged."""
+    import re
+
+    if not name.startswith("_ZN"):
+        return name
+    i, parts = 3, []
+    while i < len(name) and name[i].isdigit():
+        m = re.match(r"\d+", name[i:])
+        n = int(m.group())
+        i += len(m.group())
+        parts.append(name[i:i + n])
+        i += n
+    args = []
+    if i < len(name) and name[i] == "I":
+        for kind, val in re.findall(r"L([ib])(\d+)E", name[i:name.find("EE", i) + 1]):
+            args.append(("true" if val == "1" else "false") if kind == "b" else val)
+    base = "::".join(p for p in parts if not p.startswith("_GLOBAL__N"))
+    return "void " + base + (f"<{', '.join(args)}>" if args else "") + "()"
+
+
 def _short_name(name):
     """Kernel name without namespaces, template arguments and parameters."""
     import re
 
-    base = name.replace("(anonymous namespace)::", "").split("(")[0]
+    base = _demangle(name).replace("(anonymous namespace)::", "").split("(")[0]
     base = re.sub(r"<.*>", "", base)
     base = base.replace("void ", "").strip()
     return base.split("::")[-1] or name[:40]
 
 
-def summarize(trace_dir, fetch_dir, write_dir, method):
+def _kernel_label(name):
+    """Kernel name with its template arguments, without namespaces and
+    parameters (tells the template instances of one kernel apart)."""
+    base = _demangle(name).replace("(anonymous namespace)::", "").split("(")[0]
+    return base.replace("void ", "").replace("rgn::", "").strip() or name[:40]
+
+
+def summarize(trace_dir, fetch_dir, write_dir, method, tag=None, title=None, source=None):
     """Per-kernel durations and HBM bytes from rocprofv3 CSV output of --profile.
 
     Byte counters (FETCH_SIZE, WRITE_SIZE, KiB per dispatch) are averaged over
@@ -311,11 +325,14 @@ def summarize(trace_dir, fetch_dir, write_dir, method):
     import csv
     import statistics
 
+    # template instances apart for the compute kernels; short names for the
+    # emission kernels (the keys the bench line's traffic lookup uses)
+    label = _kernel_label if method == "compute" else _short_name
     stats = list(csv.DictReader(open(Path(trace_dir) / "run_kernel_stats.csv")))
     counters = {}
     for d, c in ((fetch_dir, "FETCH_SIZE"), (write_dir, "WRITE_SIZE")):
         for r in csv.DictReader(open(Path(d) / "run_counter_collection.csv")):
-            key = (_short_name(r["Kernel_Name"]), int(r["Grid_Size"]))
+            key = (label(r["Kernel_Name"]), int(r["Grid_Size"]))
             counters.setdefault(key, {}).setdefault(c, []).append(float(r["Counter_Value"]))
 
     def warm(v):
@@ -329,7 +346,7 @@ def summarize(trace_dir, fetch_dir, write_dir, method):
     wfac = gib_kib / warm(copy[0]["WRITE_SIZE"])
     lines = ["| kernel | dispatches | avg duration (us) | min (us) | max (us) |", "|---|---|---|---|---|"]
     for r in stats:
-        lines.append(f"| `{_short_name(r['Name'])}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
+        lines.append(f"| `{label(r['Name'])}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
                      f"{float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} |")
     tl = ["| kernel (grid threads) | FETCH_SIZE KiB | WRITE_SIZE KiB | HBM bytes / launch (scaled) |",
           "|---|---|---|---|"]
@@ -343,16 +360,19 @@ def summarize(trace_dir, fetch_dir, write_dir, method):
         if k not in traffic or b > traffic[k]["bytes_per_launch"]:
             traffic[k] = {"fetch_kib": fk, "write_kib": wk, "bytes_per_launch": b,
                           "fetch_factor": ffac, "write_factor": wfac, "grid_threads": g}
-    md = [f"# Kernel durations and HBM traffic, 2^24 particles x 256 bins, method {method}", "",
+    tag = tag or f"2p24_{method}"
+    title = title or f"2^24 particles x 256 bins, method {method}"
+    source = source or f"--profile --method {method}"
+    md = [f"# Kernel durations and HBM traffic, {title}", "",
           f"Made by `python bench.py --method {method} --summarize <trace> <fetch> <write>` from",
-          f"the three `bench.py --profile --method {method}` runs listed in profiles/README.md.", "",
+          f"the three `bench.py {source}` runs listed in profiles/README.md.", "",
           "## Durations (rocprofv3 --kernel-trace --stats)", "",
           *lines, "", "## HBM traffic (rocprofv3 --pmc, warm dispatches)", "",
           f"Scale factors from the 1 GiB copy: FETCH x{ffac:.3f}, WRITE x{wfac:.3f}.", "", *tl, ""]
-    (ROOT / "profiles" / f"kernels_2p24_{method}.md").write_text("\n".join(md))
+    (ROOT / "profiles" / f"kernels_{tag}.md").write_text("\n".join(md))
     tpath = ROOT / "profiles" / "traffic_2p24.json"
     merged = json.loads(tpath.read_text()) if tpath.exists() else {}
-    merged.update({k: v for k, v in traffic.items() if not k.startswith(("__amd", "vectorized"))})
+    merged.update({k: v for k, v in traffic.items() if not k.startswith(("__amd", "vectorized", "at::"))})
     tpath.write_text(json.dumps(merged, indent=1, sort_keys=True))
     print("\n".join(md))
 
@@ -765,7 +785,11 @@ def main():
         sys.exit(2)
 
     if args.summarize:
-        summarize(*args.summarize, args.method)
+        if args.method == "compute":
+            summarize(*args.summarize, "compute", tag="compute_kernels",
+                      title="compute-bound kernels (bench.py --profile-compute)", source="--profile-compute")
+        else:
+            summarize(*args.summarize, args.method)
         return
     if args.summarize_compute:
         summarize_compute(*args.summarize_compute)

@@ -204,11 +204,13 @@ int rgn_sync_fromdist_batched(const rgn_sync_plan* plan, const float* d_gbeta, i
 /* rgn_sync_fromdist_batched with the rows given by two numbers each instead
  * of ng values: f[d, g] = x_g^p_d exp(-x_g / xc_d) / N_d, x_g = d_gbeta[g],
  * p_d = d_index[d], xc_d = d_cutoff[d], N_d the trapezoid integral of the
- * row over the grid (sum_g (f_g + f_g+1) (x_g+1 - x_g) / 2).  The rows are
- * generated inside the split-FP16 kernel (FP64 exponent, float values), so
- * the call reads 8 bytes per distribution instead of 4 ng.  precision:
- * RGN_PREC_F16X3, RGN_PREC_F16X1 or RGN_PREC_BF16X6 (RGN_ERR_UNSUPPORTED
- * otherwise); ng >= 2.
+ * row over the grid (sum_g (f_g + f_g+1) (x_g+1 - x_g) / 2).  The rows and
+ * their norms are generated inside the matrix-core kernels (FP64 exponent,
+ * float values, FP64 norm), so the call reads 8 bytes per distribution
+ * instead of 4 ng.  precision: RGN_PREC_F16X3, RGN_PREC_F16X1 or
+ * RGN_PREC_BF16X6 (RGN_ERR_UNSUPPORTED otherwise); 2 <= ng <= 2048
+ * (RGN_ERR_INVALID below, RGN_ERR_RANGE above: the grid's FP64 terms are
+ * staged in LDS).
  * No reference counterpart (the reference evaluates a generator on the host
  * per distribution, distributions.cpp:37-66, and integrates each with
  * SynchrotronSpectrumFromDist, synchrotron.cpp:69-105). */

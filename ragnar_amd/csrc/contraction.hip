@@ -114,6 +114,7 @@ extern "C" int rgn_sync_fromdist_batched_plaw(const rgn_sync_plan* plan, const f
     RGN_REQUIRE(plan != nullptr, RGN_ERR_INVALID, "null plan");
     RGN_REQUIRE(d_gbeta && d_out && (nd == 0 || (d_index && d_cutoff)), RGN_ERR_INVALID, "null argument");
     RGN_REQUIRE(ng >= 2, RGN_ERR_INVALID, "a generated distribution needs at least two grid points");
+    RGN_REQUIRE(ng <= 2048, RGN_ERR_RANGE, "generated distributions take at most 2048 grid points");
     RGN_REQUIRE(precision == RGN_PREC_F16X3 || precision == RGN_PREC_F16X1 || precision == RGN_PREC_BF16X6,
                 RGN_ERR_UNSUPPORTED,
                 "generated rows run on the matrix-core kernels only (RGN_PREC_F16X3, _F16X1 or _BF16X6)");

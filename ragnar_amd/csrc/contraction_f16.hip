@@ -65,32 +65,58 @@ __device__ __forceinline__ size_t packed_index(int kb, int c, int i, int nbp, in
                  : (static_cast<size_t>(kb) * nbp + c) * kKB + i;
 }
 
+// One block per (16 columns, g block): the 64 x 16 tile of K is read into
+// LDS (each row segment 64 contiguous bytes), each column's block maximum sets
+// its exponent (16 partial maxima per column, then their maximum), and the
+// split halves are written in packed order, consecutive threads to
+// consecutive addresses.
+constexpr int kPackCols = 16;
+
 __global__ void __launch_bounds__(256) pack_kernel_matrix(const float* __restrict__ K, int ng, int nb, int nbp,
                                                           _Float16* __restrict__ Bh, _Float16* __restrict__ Bl,
                                                           int* __restrict__ sB, float* __restrict__ rB,
                                                           int ksmajor) {
-  const int c = blockIdx.x * 256 + threadIdx.x;
-  const int kb = blockIdx.y;
-  if (c >= nbp) return;
-  float x[kKB];
-  float m = 0.0f;
+  __shared__ float tile[kKB][kPackCols + 1];
+  __shared__ float part[16][kPackCols];
+  __shared__ int sexp[kPackCols];
+  const int c0 = blockIdx.x * kPackCols, kb = blockIdx.y;
+  const int t = threadIdx.x;
 #pragma unroll
-  for (int i = 0; i < kKB; ++i) {
-    const int g = kb * kKB + i;
-    x[i] = (c < nb && g < ng) ? K[static_cast<size_t>(g) * nb + c] : 0.0f;
-    m = fmaxf(m, fabsf(x[i]));
+  for (int r = 0; r < kKB * kPackCols / 256; ++r) {
+    const int idx = r * 256 + t, i = idx / kPackCols, c = idx % kPackCols, g = kb * kKB + i;
+    tile[i][c] = (c0 + c < nb && g < ng) ? K[static_cast<size_t>(g) * nb + c0 + c] : 0.0f;
   }
-  const int s = block_exp(m);
+  __syncthreads();
+  {
+    const int c = t % kPackCols, q = t / kPackCols;  // rows 4q .. 4q + 3
+    float m = 0.0f;
 #pragma unroll
-  for (int i = 0; i < kKB; ++i) {
+    for (int i = 0; i < 4; ++i) m = fmaxf(m, fabsf(tile[4 * q + i][c]));
+    part[q][c] = m;
+  }
+  __syncthreads();
+  if (t < kPackCols) {
+    float m = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) m = fmaxf(m, part[q][t]);
+    const int sc = block_exp(m);
+    sexp[t] = sc;
+    sB[static_cast<size_t>(kb) * nbp + c0 + t] = sc;
+    rB[static_cast<size_t>(kb) * nbp + c0 + t] = ldexpf(1.0f, -sc);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kKB * kPackCols / 256; ++r) {
+    const int idx = r * 256 + t;
+    // step-major: (16-g step, column, g within the step); block-major: (column, g)
+    const int c = ksmajor ? (idx / 16) % kPackCols : idx / kKB;
+    const int i = ksmajor ? (idx / (16 * kPackCols)) * 16 + idx % 16 : idx % kKB;
     _Float16 hi, lo;
-    split2(x[i], s, hi, lo);
-    const size_t at = packed_index(kb, c, i, nbp, ksmajor);
+    split2(tile[i][c], sexp[c], hi, lo);
+    const size_t at = packed_index(kb, c0 + c, i, nbp, ksmajor);
     Bh[at] = hi;
     Bl[at] = lo;
   }
-  sB[static_cast<size_t>(kb) * nbp + c] = s;
-  rB[static_cast<size_t>(kb) * nbp + c] = ldexpf(1.0f, -s);
 }
 
 template <int NPROD, bool VEC>
@@ -253,28 +279,46 @@ __global__ void __launch_bounds__(kThreadsC) mfma_contract(const float* __restri
 // Each wave owns 64 columns (2 x 2 MFMA tiles over all 64 rows).  Only the
 // f tile is shared by the waves, so only it goes through LDS, double
 // buffered: while the matrix cores work on block kb, the waves split block
-// kb + 1 (already in registers) into the other buffer and load block kb + 2,
-// and one barrier per block orders the buffers.  K fragments are read by each
-// wave straight from the packed layout (1 MB for 512 x 512, resident in L2),
-// one 16-g step ahead.  The two workgroups of a CU meet at their own
-// barriers only, so one's rescale overlaps the other's matrix work.  Row and
-// column scales are floats (2^-s), so the rescale is one multiply and one FMA
-// per element.  Workgroups are numbered so that the column tiles of a row
-// block run on one XCD (one L2) at about the same time, and f is fetched from
-// HBM once.
+// kb + 1 (already in registers) into the other buffer, and one barrier per
+// block orders the buffers.  K fragments are read by each wave straight from
+// the packed layout (1 MB for 512 x 512, resident in L2), one 16-g step
+// ahead.  The two workgroups of a CU meet at their own barriers only, so one's
+// rescale overlaps the other's matrix work.  Row and column scales are floats
+// (2^-s), so the rescale is one multiply and one FMA per element.  Workgroups
+// are numbered so that the column tiles of a row block run on one XCD (one
+// L2) at about the same time, and f is fetched from HBM once.
+//
+// The block loop has no branches: the split of the next block always runs
+// (after the last block it fills a buffer nobody reads), loads past the last
+// step or block re-read the last one, and the f load of block kb + 2 is
+// issued after the block's last K load.  Loads complete in order, so a K
+// fragment issued after a load of f cannot be used before that f arrives
+// from HBM; issued last, the f load has the rescale, the barrier and the next
+// block's first step to arrive in.  The column scales of a block are loaded
+// at its start and used at its end.
 constexpr int kTM2 = 64;
 constexpr int kThreads2 = 256;
 
-// Rows of f given as terms instead of values (GEN): f[d, g] = 2^t with
-// t = p_d L_g - c_d x_g + o_d, L_g = log2 x_g, c_d = log2(e) / x_cut,d and
-// o_d = -log2 of the row's trapezoid norm (plaw_rows); t is formed in FP64
-// and 2^t as 2^n exp2f(t - n), n = floor(t), so f is within ~2 float ulps of
-// the same expression evaluated in FP64 and rounded.
+// Rows of f given as terms instead of values (GEN): a power law with an
+// exponential cutoff over the grid x_g,
+//     f[d, g] = x_g^p_d e^(-x_g / xc_d) / N_d,
+// N_d its trapezoid integral over the grid.  The kernels stage L_g = log2 x_g,
+// x_g and the trapezoid weights w_g = (x_g+1 - x_g-1) / 2 (halved end
+// intervals) in LDS, form t = p_d L_g - c_d x_g - t_max,d in FP64 (c_d =
+// log2(e) / xc_d) and the value 2^t as 2^n exp2f(t - n), n = floor(t), within
+// ~2 float ulps of the same expression in FP64, and accumulate N_d 2^-t_max,d
+// = sum_g f_g w_g in FP64 from the same float values while generating them;
+// the row's results are divided by it when stored.  t(x) = p log2 x - c x is
+// convex in x for p < 0 (largest at an end of the grid) and concave for
+// p >= 0 (largest at x* = p / (c ln 2) when x* lies inside the grid, else at
+// an end), so t_max, the larger of the end values and t(x*), bounds every
+// exponent: no value overflows and the largest is 1.
 struct GenRows {
-  const double* rows;  // [nd][3]: p_d, c_d, o_d
-  const double* L;     // [ng] log2 x_g
-  const double* x;     // [ng] x_g
+  const float* index;   // [nd] p_d
+  const float* cutoff;  // [nd] xc_d
+  const float* grid;    // [ng] x_g
 };
+constexpr int kGenMaxNg = 2048;  // three FP64 grid terms per g in LDS
 
 __device__ __forceinline__ float gen_value(double p, double c, double o, double L, double x) {
   const double t = fma(p, L, fma(-c, x, o));
@@ -284,7 +328,98 @@ __device__ __forceinline__ float gen_value(double p, double c, double o, double 
   return ldexpf(__builtin_amdgcn_exp2f(static_cast<float>(t - n)), static_cast<int>(n));
 }
 
-template <int NPROD, bool VEC, bool GEN = false>
+// Grid terms of generated rows into LDS: L_g, x_g, w_g (three arrays of ng).
+__device__ __forceinline__ void stage_grid_terms(const float* __restrict__ grid, int ng, double* gterms) {
+  for (int g = threadIdx.x; g < ng; g += blockDim.x) {
+    const double x = grid[g];
+    gterms[g] = log2(x);
+    gterms[ng + g] = x;
+    gterms[2 * ng + g] = 0.5 * (static_cast<double>(grid[min(g + 1, ng - 1)]) - static_cast<double>(grid[max(g - 1, 0)]));
+  }
+}
+
+// f rows as the matrix kernels read them: thread (row ar, quarter aq) of a
+// 256-thread workgroup holds 16 consecutive g of row d0 + ar of one g block.
+//   LOAD 0  scalar loads, each element checked against nd and ng;
+//   LOAD 1  16-byte loads (rows 16-byte aligned, ng % 4 == 0), each checked;
+//   LOAD 2  16-byte loads of whole blocks (and ng % 64 == 0): rows past nd
+//           read row nd - 1, whose results are computed but never stored;
+//   GEN     generated (GenRows; grid terms staged in LDS by the kernel),
+//           rows past nd generate row nd - 1.
+// load(kb, ra, count): block kb into ra; with count (GEN), the block's
+// values also go into the row's norm (each block is counted once, though the
+// kernels' branch-free loops load the last block more than once).
+template <int LOAD, bool GEN>
+struct RowBlocks {
+  const float* row;      // LOAD 2: start of this thread's row and quarter
+  const float* A;
+  int nd, ng, d, g_off;  // g_off = 16 aq
+  const double* gL;      // GEN: LDS grid terms L_g, x_g, w_g
+  double gp, gc, go, nsum;
+
+  __device__ RowBlocks(const float* A_, int nd_, int ng_, int d_, int aq, const GenRows& gen, const double* gterms)
+      : row(nullptr), A(A_), nd(nd_), ng(ng_), d(d_), g_off(16 * aq), gL(gterms), gp(0.0), gc(0.0), go(0.0),
+        nsum(0.0) {
+    const int dc = min(d, nd - 1);
+    if (LOAD == 2 && !GEN) row = A + static_cast<size_t>(dc) * ng + g_off;
+    if (GEN) {
+      gp = gen.index[dc];
+      gc = 1.4426950408889634 / static_cast<double>(gen.cutoff[dc]);  // log2(e) / xc
+      const double* gx = gL + ng;
+      double tmax = fmax(fma(gp, gL[0], -gc * gx[0]), fma(gp, gL[ng - 1], -gc * gx[ng - 1]));
+      const double xs = gp / (gc * 0.69314718055994530942);
+      if (xs > gx[0] && xs < gx[ng - 1]) tmax = fmax(tmax, fma(gp, log2(xs), -gc * xs));
+      go = -tmax;
+    }
+  }
+
+  __device__ __forceinline__ void load(int kb, float (&ra)[16], bool count) {
+    const int g0 = kb * kKB + g_off;
+    if (GEN) {
+      double s = 0.0;
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const bool in = g0 + i < ng;
+        ra[i] = in ? gen_value(gp, gc, go, gL[g0 + i], gL[ng + g0 + i]) : 0.0f;
+        s = fma(static_cast<double>(ra[i]), in ? gL[2 * ng + g0 + i] : 0.0, s);
+      }
+      if (count) nsum += s;
+    } else if (LOAD == 2) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const float4 v = *reinterpret_cast<const float4*>(row + kb * kKB + 4 * k);
+        ra[4 * k] = v.x;
+        ra[4 * k + 1] = v.y;
+        ra[4 * k + 2] = v.z;
+        ra[4 * k + 3] = v.w;
+      }
+    } else if (LOAD == 1) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (d < nd && g0 + 4 * k < ng) v = *reinterpret_cast<const float4*>(A + static_cast<size_t>(d) * ng + g0 + 4 * k);
+        ra[4 * k] = v.x;
+        ra[4 * k + 1] = v.y;
+        ra[4 * k + 2] = v.z;
+        ra[4 * k + 3] = v.w;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 16; ++i) ra[i] = (d < nd && g0 + i < ng) ? A[static_cast<size_t>(d) * ng + g0 + i] : 0.0f;
+    }
+  }
+
+  // GEN: 1 / N_d of this thread's row (the row's four threads each hold a
+  // quarter of the sum; every thread of the workgroup calls this)
+  __device__ __forceinline__ float inv_norm() const {
+    double t = nsum;
+    t += __shfl_xor(t, 1);
+    t += __shfl_xor(t, 2);
+    return static_cast<float>(1.0 / t);
+  }
+};
+
+template <int NPROD, int LOAD, bool GEN = false>
 __global__ void __launch_bounds__(kThreads2, 2) mfma_contract2(const float* __restrict__ A, int nd, int ng,
                                                                const _Float16* __restrict__ Bh,
                                                                const _Float16* __restrict__ Bl,
@@ -304,45 +439,14 @@ __global__ void __launch_bounds__(kThreads2, 2) mfma_contract2(const float* __re
   const int wc = wave * 64;
   // f producer: four threads per row, 16 consecutive g each
   const int ar = threadIdx.x >> 2, aq = threadIdx.x & 3;
-  const int d = d0 + ar;
-  float ra[16];
-  double gp = 0.0, gc = 0.0, go = 0.0;
-  // generated rows: the grid terms L_g, x_g in LDS (read by every row)
-  extern __shared__ __align__(16) double gterms[];  // GEN: [2][ng]
+  // generated rows: the grid terms in LDS (read by every row)
+  extern __shared__ __align__(16) double gterms[];  // GEN: [3][ng]
   if (GEN) {
-    for (int i = threadIdx.x; i < ng; i += blockDim.x) {
-      gterms[i] = gen.L[i];
-      gterms[ng + i] = gen.x[i];
-    }
+    stage_grid_terms(gen.grid, ng, gterms);
     __syncthreads();
-    if (d < nd) {
-      gp = gen.rows[3 * static_cast<size_t>(d)];
-      gc = gen.rows[3 * static_cast<size_t>(d) + 1];
-      go = gen.rows[3 * static_cast<size_t>(d) + 2];
-    }
   }
-  auto load_a = [&](int kb) {
-    const int g0 = kb * kKB + aq * 16;
-    if (GEN) {
-#pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        ra[i] = (d < nd && g0 + i < ng) ? gen_value(gp, gc, go, gterms[g0 + i], gterms[ng + g0 + i]) : 0.0f;
-      }
-    } else if (VEC) {
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (d < nd && g0 + 4 * k < ng) v = *reinterpret_cast<const float4*>(A + static_cast<size_t>(d) * ng + g0 + 4 * k);
-        ra[4 * k] = v.x;
-        ra[4 * k + 1] = v.y;
-        ra[4 * k + 2] = v.z;
-        ra[4 * k + 3] = v.w;
-      }
-    } else {
-#pragma unroll
-      for (int i = 0; i < 16; ++i) ra[i] = (d < nd && g0 + i < ng) ? A[static_cast<size_t>(d) * ng + g0 + i] : 0.0f;
-    }
-  };
+  RowBlocks<LOAD, GEN> src(A, nd, ng, d0 + ar, aq, gen, gterms);
+  float ra[16];
   auto split_a = [&](int buf) {
     float m = 0.0f;
 #pragma unroll
@@ -350,7 +454,7 @@ __global__ void __launch_bounds__(kThreads2, 2) mfma_contract2(const float* __re
     m = fmaxf(m, __shfl_xor(m, 1));
     m = fmaxf(m, __shfl_xor(m, 2));
     const int sc = block_exp(m);
-    if (aq == 0) rA[buf][ar] = ldexpf(1.0f, -sc);
+    rA[buf][ar] = ldexpf(1.0f, -sc);  // the row's four threads store the same value
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
       half8 h, l;
@@ -366,10 +470,12 @@ __global__ void __launch_bounds__(kThreads2, 2) mfma_contract2(const float* __re
     }
   };
   // K fragments of this wave (step-major layout): column wc + 32 j + lane % 32,
-  // g = 16 ks + 8 (lane / 32) .. + 8
+  // g = 16 s + 8 (lane / 32) .. + 8 of step s; column tile j + 1 is 512 halves on
   const int kofs = (lane >> 5) * 8;
-  auto b_addr = [&](int kb, int j, int ks) {
-    return packed_index(kb, c0 + wc + j * 32 + (lane & 31), ks * 16 + kofs, nbp, 1);
+  const size_t b0 = packed_index(0, c0 + wc + (lane & 31), kofs, nbp, 1);
+  const int nsteps = nkb * (kKB / 16);
+  auto ldb = [&](const _Float16* B, int s, int j) {
+    return *reinterpret_cast<const half8*>(B + b0 + static_cast<size_t>(s) * nbp * 16 + j * 512);
   };
   float16v acc[2][2];
 #pragma unroll
@@ -380,18 +486,21 @@ __global__ void __launch_bounds__(kThreads2, 2) mfma_contract2(const float* __re
       for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
     }
   }
-  load_a(0);
+  src.load(0, ra, true);
   split_a(0);
-  if (nkb > 1) load_a(1);
+  src.load(min(1, nkb - 1), ra, nkb > 1);
   half8 bh[2], bl[2];
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
-    bh[j] = *reinterpret_cast<const half8*>(Bh + b_addr(0, j, 0));
-    if (NPROD == 3) bl[j] = *reinterpret_cast<const half8*>(Bl + b_addr(0, j, 0));
+    bh[j] = ldb(Bh, 0, j);
+    if (NPROD == 3) bl[j] = ldb(Bl, 0, j);
   }
   __syncthreads();
   for (int kb = 0; kb < nkb; ++kb) {
     const int cur = kb & 1;
+    float sc[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) sc[j] = rB[static_cast<size_t>(kb) * nbp + c0 + wc + j * 32 + (lane & 31)];
     float16v tmp[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
@@ -404,15 +513,12 @@ __global__ void __launch_bounds__(kThreads2, 2) mfma_contract2(const float* __re
 #pragma unroll
     for (int ks = 0; ks < kKB / 16; ++ks) {
       // next step's K fragments (the next block's first step after the last)
+      const int sn = min(kb * (kKB / 16) + ks + 1, nsteps - 1);
       half8 nbh[2], nbl[2];
-      const int nks = (ks + 1) % (kKB / 16);
-      const int nkbb = ks + 1 < kKB / 16 ? kb : kb + 1;
-      if (nkbb < nkb) {
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          nbh[j] = *reinterpret_cast<const half8*>(Bh + b_addr(nkbb, j, nks));
-          if (NPROD == 3) nbl[j] = *reinterpret_cast<const half8*>(Bl + b_addr(nkbb, j, nks));
-        }
+      for (int j = 0; j < 2; ++j) {
+        nbh[j] = ldb(Bh, sn, j);
+        if (NPROD == 3) nbl[j] = ldb(Bl, sn, j);
       }
       const int kk = ks * 16 + kofs;
       half8 ah[2], al[2];
@@ -432,37 +538,38 @@ __global__ void __launch_bounds__(kThreads2, 2) mfma_contract2(const float* __re
           }
         }
       }
-      if (nkbb < nkb) {
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          bh[j] = nbh[j];
-          if (NPROD == 3) bl[j] = nbl[j];
-        }
+      for (int j = 0; j < 2; ++j) {
+        bh[j] = nbh[j];
+        if (NPROD == 3) bl[j] = nbl[j];
       }
-      // the next block's f: split into the other buffer, then fetch the one after
-      if (ks == 1 && kb + 1 < nkb) {
-        split_a(cur ^ 1);
-        if (kb + 2 < nkb) load_a(kb + 2);
-      }
+      if (ks == 1) split_a(cur ^ 1);                                  // the next block's f
+      if (ks == kKB / 16 - 1) src.load(min(kb + 2, nkb - 1), ra, kb + 2 < nkb);  // the one after
     }
     // acc += tmp 2^-(s_row + s_col); output element v of a lane sits at
     // row 8 (v / 4) + 4 (lane / 32) + v % 4, column lane % 32 of its 32x32 tile
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-      const float sc = rB[static_cast<size_t>(kb) * nbp + c0 + wc + j * 32 + (lane & 31)];
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const float4 sr = *reinterpret_cast<const float4*>(&rA[cur][i * 32 + 8 * r + 4 * (lane >> 5)]);
-          acc[i][j][4 * r + 0] = fmaf(tmp[i][j][4 * r + 0] * sr.x, sc, acc[i][j][4 * r + 0]);
-          acc[i][j][4 * r + 1] = fmaf(tmp[i][j][4 * r + 1] * sr.y, sc, acc[i][j][4 * r + 1]);
-          acc[i][j][4 * r + 2] = fmaf(tmp[i][j][4 * r + 2] * sr.z, sc, acc[i][j][4 * r + 2]);
-          acc[i][j][4 * r + 3] = fmaf(tmp[i][j][4 * r + 3] * sr.w, sc, acc[i][j][4 * r + 3]);
+          acc[i][j][4 * r + 0] = fmaf(tmp[i][j][4 * r + 0] * sr.x, sc[j], acc[i][j][4 * r + 0]);
+          acc[i][j][4 * r + 1] = fmaf(tmp[i][j][4 * r + 1] * sr.y, sc[j], acc[i][j][4 * r + 1]);
+          acc[i][j][4 * r + 2] = fmaf(tmp[i][j][4 * r + 2] * sr.z, sc[j], acc[i][j][4 * r + 2]);
+          acc[i][j][4 * r + 3] = fmaf(tmp[i][j][4 * r + 3] * sr.w, sc[j], acc[i][j][4 * r + 3]);
         }
       }
     }
     __syncthreads();  // buffer cur is free; buffer cur ^ 1 is written
+  }
+  // generated rows: divide by the row norms (via LDS, like the row scales)
+  __shared__ float rnorm[GEN ? kTM2 : 1];
+  if (GEN) {
+    const float r = src.inv_norm();
+    rnorm[ar] = r;  // the row's four threads store the same value
+    __syncthreads();
   }
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
@@ -471,8 +578,10 @@ __global__ void __launch_bounds__(kThreads2, 2) mfma_contract2(const float* __re
       const int c = c0 + wc + j * 32 + (lane & 31);
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
-        const int dd = d0 + i * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-        if (dd < nd && c < nb) __builtin_nontemporal_store(acc[i][j][v], out + static_cast<size_t>(dd) * nb + c);
+        const int rr = i * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+        const int dd = d0 + rr;
+        const float val = GEN ? acc[i][j][v] * rnorm[rr] : acc[i][j][v];
+        if (dd < nd && c < nb) __builtin_nontemporal_store(val, out + static_cast<size_t>(dd) * nb + c);
       }
     }
   }
@@ -501,26 +610,28 @@ __device__ __forceinline__ void split3(float x, __bf16& h, __bf16& m, __bf16& l)
   l = static_cast<__bf16>(r1 - static_cast<float>(m));  // exact difference
 }
 
-// K[g][e] -> B{h,m,l} in the step-major packed layout.
+// K[g][e] -> B{h,m,l} in the step-major packed layout: one thread per packed
+// element, in output order.
 __global__ void __launch_bounds__(256) pack_kernel_matrix_bf16(const float* __restrict__ K, int ng, int nb, int nbp,
-                                                               __bf16* __restrict__ Bh, __bf16* __restrict__ Bm,
-                                                               __bf16* __restrict__ Bl) {
-  const int c = blockIdx.x * 256 + threadIdx.x;
-  const int kb = blockIdx.y;
-  if (c >= nbp) return;
-  for (int i = 0; i < kKB; ++i) {
-    const int g = kb * kKB + i;
-    const float x = (c < nb && g < ng) ? K[static_cast<size_t>(g) * nb + c] : 0.0f;
-    __bf16 h, m, l;
-    split3(x, h, m, l);
-    const size_t at = packed_index(kb, c, i, nbp, 1);
-    Bh[at] = h;
-    Bm[at] = m;
-    Bl[at] = l;
-  }
+                                                               size_t packed, __bf16* __restrict__ Bh,
+                                                               __bf16* __restrict__ Bm, __bf16* __restrict__ Bl) {
+  const size_t at = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (at >= packed) return;
+  // at = ((kb * 4 + ks) * nbp + c) * 16 + i16
+  const int i16 = static_cast<int>(at % 16);
+  const size_t r = at / 16;
+  const int c = static_cast<int>(r % nbp);
+  const int step = static_cast<int>(r / nbp);
+  const int g = step * 16 + i16;
+  const float x = (c < nb && g < ng) ? K[static_cast<size_t>(g) * nb + c] : 0.0f;
+  __bf16 h, m, l;
+  split3(x, h, m, l);
+  Bh[at] = h;
+  Bm[at] = m;
+  Bl[at] = l;
 }
 
-template <bool VEC, bool GEN>
+template <int LOAD, bool GEN>
 __global__ void __launch_bounds__(kThreads2, 2) mfma_contract_bf16(const float* __restrict__ A, int nd, int ng,
                                                                    const __bf16* __restrict__ Bh,
                                                                    const __bf16* __restrict__ Bm,
@@ -536,45 +647,14 @@ __global__ void __launch_bounds__(kThreads2, 2) mfma_contract_bf16(const float* 
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int wc = wave * 64;
   const int ar = threadIdx.x >> 2, aq = threadIdx.x & 3;
-  const int d = d0 + ar;
-  float ra[16];
-  double gp = 0.0, gc = 0.0, go = 0.0;
-  // generated rows: the grid terms L_g, x_g in LDS (read by every row)
-  extern __shared__ __align__(16) double gterms[];  // GEN: [2][ng]
+  // generated rows: the grid terms in LDS (read by every row)
+  extern __shared__ __align__(16) double gterms[];  // GEN: [3][ng]
   if (GEN) {
-    for (int i = threadIdx.x; i < ng; i += blockDim.x) {
-      gterms[i] = gen.L[i];
-      gterms[ng + i] = gen.x[i];
-    }
+    stage_grid_terms(gen.grid, ng, gterms);
     __syncthreads();
-    if (d < nd) {
-      gp = gen.rows[3 * static_cast<size_t>(d)];
-      gc = gen.rows[3 * static_cast<size_t>(d) + 1];
-      go = gen.rows[3 * static_cast<size_t>(d) + 2];
-    }
   }
-  auto load_a = [&](int kb) {
-    const int g0 = kb * kKB + aq * 16;
-    if (GEN) {
-#pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        ra[i] = (d < nd && g0 + i < ng) ? gen_value(gp, gc, go, gterms[g0 + i], gterms[ng + g0 + i]) : 0.0f;
-      }
-    } else if (VEC) {
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (d < nd && g0 + 4 * k < ng) v = *reinterpret_cast<const float4*>(A + static_cast<size_t>(d) * ng + g0 + 4 * k);
-        ra[4 * k] = v.x;
-        ra[4 * k + 1] = v.y;
-        ra[4 * k + 2] = v.z;
-        ra[4 * k + 3] = v.w;
-      }
-    } else {
-#pragma unroll
-      for (int i = 0; i < 16; ++i) ra[i] = (d < nd && g0 + i < ng) ? A[static_cast<size_t>(d) * ng + g0 + i] : 0.0f;
-    }
-  };
+  RowBlocks<LOAD, GEN> src(A, nd, ng, d0 + ar, aq, gen, gterms);
+  float ra[16];
   auto split_a = [&](int buf) {
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
@@ -593,9 +673,8 @@ __global__ void __launch_bounds__(kThreads2, 2) mfma_contract_bf16(const float* 
     }
   };
   const int kofs = (lane >> 5) * 8;
-  auto b_addr = [&](int kb, int j, int ks) {
-    return packed_index(kb, c0 + wc + j * 32 + (lane & 31), ks * 16 + kofs, nbp, 1);
-  };
+  const size_t b0 = packed_index(0, c0 + wc + (lane & 31), kofs, nbp, 1);
+  const int nsteps = nkb * (kKB / 16);
   float16v acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
@@ -605,20 +684,19 @@ __global__ void __launch_bounds__(kThreads2, 2) mfma_contract_bf16(const float* 
       for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
     }
   }
-  load_a(0);
+  src.load(0, ra, true);
   split_a(0);
-  if (nkb > 1) load_a(1);
-  // K fragments two 16-g steps ahead: b? holds step s, n1? step s + 1
+  src.load(min(1, nkb - 1), ra, nkb > 1);
+  // K fragments two 16-g steps ahead: b? holds step s, n1? step s + 1; steps
+  // past the last re-read the last
   bf16x8 bh[2], bm[2], bl[2], n1h[2], n1m[2], n1l[2];
   auto load_b = [&](int step, bf16x8* h, bf16x8* m, bf16x8* l) {
-    const int kb = step / (kKB / 16), ks = step % (kKB / 16);
-    if (kb < nkb) {
+    const size_t at = b0 + static_cast<size_t>(min(step, nsteps - 1)) * nbp * 16;
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        h[j] = *reinterpret_cast<const bf16x8*>(Bh + b_addr(kb, j, ks));
-        m[j] = *reinterpret_cast<const bf16x8*>(Bm + b_addr(kb, j, ks));
-        l[j] = *reinterpret_cast<const bf16x8*>(Bl + b_addr(kb, j, ks));
-      }
+    for (int j = 0; j < 2; ++j) {
+      h[j] = *reinterpret_cast<const bf16x8*>(Bh + at + j * 512);
+      m[j] = *reinterpret_cast<const bf16x8*>(Bm + at + j * 512);
+      l[j] = *reinterpret_cast<const bf16x8*>(Bl + at + j * 512);
     }
   };
   load_b(0, bh, bm, bl);
@@ -660,12 +738,17 @@ __global__ void __launch_bounds__(kThreads2, 2) mfma_contract_bf16(const float* 
         n1m[j] = nbm[j];
         n1l[j] = nbl[j];
       }
-      if (ks == 1 && kb + 1 < nkb) {
-        split_a(cur ^ 1);
-        if (kb + 2 < nkb) load_a(kb + 2);
-      }
+      if (ks == 1) split_a(cur ^ 1);                                  // the next block's f
+      if (ks == kKB / 16 - 1) src.load(min(kb + 2, nkb - 1), ra, kb + 2 < nkb);  // the one after
     }
     __syncthreads();  // buffer cur is free; buffer cur ^ 1 is written
+  }
+  // generated rows: divide by the row norms (via LDS, like the row scales)
+  __shared__ float rnorm[GEN ? kTM2 : 1];
+  if (GEN) {
+    const float r = src.inv_norm();
+    rnorm[ar] = r;  // the row's four threads store the same value
+    __syncthreads();
   }
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
@@ -674,54 +757,12 @@ __global__ void __launch_bounds__(kThreads2, 2) mfma_contract_bf16(const float* 
       const int c = c0 + wc + j * 32 + (lane & 31);
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
-        const int dd = d0 + i * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-        if (dd < nd && c < nb) __builtin_nontemporal_store(acc[i][j][v], out + static_cast<size_t>(dd) * nb + c);
+        const int rr = i * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+        const int dd = d0 + rr;
+        const float val = GEN ? acc[i][j][v] * rnorm[rr] : acc[i][j][v];
+        if (dd < nd && c < nb) __builtin_nontemporal_store(val, out + static_cast<size_t>(dd) * nb + c);
       }
     }
-  }
-}
-
-// L_g = log2 x_g and x_g in FP64 for the generated rows.
-__global__ void __launch_bounds__(256) grid_terms(const float* __restrict__ gbeta, int ng, double* __restrict__ L,
-                                                  double* __restrict__ x) {
-  const int g = blockIdx.x * 256 + threadIdx.x;
-  if (g >= ng) return;
-  const double v = gbeta[g];
-  x[g] = v;
-  L[g] = log2(v);
-}
-
-// Per-row terms (p_d, c_d, o_d) of f[d, g] = x_g^p_d e^(-x_g / x_cut,d) / N_d,
-// N_d = sum_g (f_g + f_g+1) (x_g+1 - x_g) / 2 (trapezoid over the grid).  One
-// wave per row: each lane takes a run of intervals; the maximum exponent is
-// found first so that the summed terms 2^(t - t_max) never overflow.
-__global__ void __launch_bounds__(256) plaw_rows(const float* __restrict__ index, const float* __restrict__ cutoff,
-                                                 size_t nd, const double* __restrict__ L,
-                                                 const double* __restrict__ x, int ng, double* __restrict__ rows) {
-  const size_t d = static_cast<size_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (d >= nd) return;  // uniform per wave
-  const double p = index[d];
-  const double c = 1.4426950408889634 / static_cast<double>(cutoff[d]);  // log2(e) / x_cut
-  const int per = (ng - 1 + 63) / 64;
-  const int i0 = min(ng - 1, lane * per), i1 = min(ng - 1, i0 + per);
-  double tmax = -1e300;
-  for (int g = i0; g <= i1 && g < ng; ++g) tmax = fmax(tmax, fma(p, L[g], -c * x[g]));
-#pragma unroll
-  for (int m = 1; m < 64; m <<= 1) tmax = fmax(tmax, __shfl_xor(tmax, m));
-  double sum = 0.0;
-  double prev = (i0 < i1) ? gen_value(p, c, -tmax, L[i0], x[i0]) : 0.0;
-  for (int g = i0; g < i1; ++g) {
-    const double next = gen_value(p, c, -tmax, L[g + 1], x[g + 1]);
-    sum += 0.5 * (prev + next) * (x[g + 1] - x[g]);
-    prev = next;
-  }
-#pragma unroll
-  for (int m = 1; m < 64; m <<= 1) sum += __shfl_xor(sum, m);
-  if (lane == 0) {
-    rows[3 * d] = p;
-    rows[3 * d + 1] = c;
-    rows[3 * d + 2] = -(tmax + log2(sum));
   }
 }
 
@@ -741,8 +782,7 @@ void contraction_f16(const SyncPlan& plan, const float* d_k, int ng, const float
   int* sB = reinterpret_cast<int*>(Bl + packed);
   float* rB = reinterpret_cast<float*>(sB + static_cast<size_t>(nkb) * nbp);
   const bool form1 = env_int("RAGNAR_CONTRACT_FORM", 2) == 1;
-  pack_kernel_matrix<<<dim3(nbp / 256 + (nbp % 256 != 0), nkb), 256, 0, st>>>(d_k, ng, nb, nbp, Bh, Bl, sB, rB,
-                                                                              form1 ? 0 : 1);
+  pack_kernel_matrix<<<dim3(nbp / kPackCols, nkb), 256, 0, st>>>(d_k, ng, nb, nbp, Bh, Bl, sB, rB, form1 ? 0 : 1);
   RGN_HIP(hipGetLastError());
   stage_mark(st, "pack_kernel_matrix");
   // 16-byte loads of f when every row starts 16-byte aligned
@@ -761,41 +801,29 @@ void contraction_f16(const SyncPlan& plan, const float* d_k, int ng, const float
   } else {
     const int ntn = nbp / kTN;
     const unsigned grid = static_cast<unsigned>(ceil_div(nd, kTM2) * ntn);
-#define RGN_CONTRACT(P_, V_) \
-  mfma_contract2<P_, V_><<<grid, kThreads2, 0, st>>>(d_f, nd_i, ng, Bh, Bl, rB, nb, nbp, nkb, ntn, d_out)
-    if (nprod == 3) {
-      if (vec) RGN_CONTRACT(3, true); else RGN_CONTRACT(3, false);
-    } else {
-      if (vec) RGN_CONTRACT(1, true); else RGN_CONTRACT(1, false);
-    }
+    const int load = vec ? (ng % kKB == 0 ? 2 : 1) : 0;
+#define RGN_CONTRACT(P_, L_) \
+  mfma_contract2<P_, L_><<<grid, kThreads2, 0, st>>>(d_f, nd_i, ng, Bh, Bl, rB, nb, nbp, nkb, ntn, d_out)
+#define RGN_CONTRACT_P(P_)                 \
+  do {                                     \
+    if (load == 2) RGN_CONTRACT(P_, 2);    \
+    else if (load == 1) RGN_CONTRACT(P_, 1); \
+    else RGN_CONTRACT(P_, 0);              \
+  } while (0)
+    if (nprod == 3) RGN_CONTRACT_P(3);
+    else RGN_CONTRACT_P(1);
+#undef RGN_CONTRACT_P
 #undef RGN_CONTRACT
   }
   RGN_HIP(hipGetLastError());
   stage_mark(st, "mfma_contract");
 }
 
-// Grid and row terms of generated rows (scratch slot 18): L_g, x_g and
-// (p_d, c_d, o_d).
-static GenRows prepare_gen_rows(const float* d_gbeta, int ng, const float* d_index, const float* d_cutoff,
-                                size_t nd, hipStream_t st) {
-  double* terms = static_cast<double*>(scratch(sizeof(double) * (2 * static_cast<size_t>(ng) + 3 * nd), 18));
-  double* L = terms;
-  double* x = terms + ng;
-  double* rows = terms + 2 * static_cast<size_t>(ng);
-  grid_terms<<<static_cast<unsigned>(ceil_div(ng, 256)), 256, 0, st>>>(d_gbeta, ng, L, x);
-  RGN_HIP(hipGetLastError());
-  if (nd > 0) {
-    plaw_rows<<<static_cast<unsigned>(ceil_div(nd, 4)), 256, 0, st>>>(d_index, d_cutoff, nd, L, x, ng, rows);
-    RGN_HIP(hipGetLastError());
-    stage_mark(st, "plaw_rows");
-  }
-  return GenRows{rows, L, x};
-}
-
 void contraction_f16_plaw(const SyncPlan& plan, const float* d_k, int ng, const float* d_gbeta, const float* d_index,
                           const float* d_cutoff, size_t nd, int nprod, float* d_out, hipStream_t st) {
   RGN_REQUIRE(nprod == 1 || nprod == 3, RGN_ERR_INVALID, "split-FP16 products must be 1 or 3");
   RGN_REQUIRE(ng >= 2, RGN_ERR_INVALID, "a generated distribution needs at least two grid points");
+  RGN_REQUIRE(ng <= kGenMaxNg, RGN_ERR_RANGE, "generated distributions take at most 2048 grid points");
   RGN_REQUIRE(nd <= static_cast<size_t>(INT32_MAX) - kTM, RGN_ERR_RANGE, "too many distributions");
   const int nb = plan.nbins;
   const int nbp = static_cast<int>(ceil_div(nb, kTN)) * kTN;
@@ -806,20 +834,20 @@ void contraction_f16_plaw(const SyncPlan& plan, const float* d_k, int ng, const 
   _Float16* Bl = Bh + packed;
   int* sB = reinterpret_cast<int*>(Bl + packed);
   float* rB = reinterpret_cast<float*>(sB + static_cast<size_t>(nkb) * nbp);
-  pack_kernel_matrix<<<dim3(nbp / 256 + (nbp % 256 != 0), nkb), 256, 0, st>>>(d_k, ng, nb, nbp, Bh, Bl, sB, rB, 1);
+  pack_kernel_matrix<<<dim3(nbp / kPackCols, nkb), 256, 0, st>>>(d_k, ng, nb, nbp, Bh, Bl, sB, rB, 1);
   RGN_HIP(hipGetLastError());
   stage_mark(st, "pack_kernel_matrix");
-  const GenRows gen = prepare_gen_rows(d_gbeta, ng, d_index, d_cutoff, nd, st);
-  const size_t gsmem = 2 * sizeof(double) * static_cast<size_t>(ng);
   if (nd == 0) return;
+  const GenRows gen{d_index, d_cutoff, d_gbeta};
+  const size_t gsmem = 3 * sizeof(double) * static_cast<size_t>(ng);
   const int ntn = nbp / kTN;
   const unsigned grid = static_cast<unsigned>(ceil_div(nd, kTM2) * ntn);
   const int nd_i = static_cast<int>(nd);
   if (nprod == 3) {
-    mfma_contract2<3, false, true><<<grid, kThreads2, gsmem, st>>>(nullptr, nd_i, ng, Bh, Bl, rB, nb, nbp, nkb,
+    mfma_contract2<3, 0, true><<<grid, kThreads2, gsmem, st>>>(nullptr, nd_i, ng, Bh, Bl, rB, nb, nbp, nkb,
                                                                    ntn, d_out, gen);
   } else {
-    mfma_contract2<1, false, true><<<grid, kThreads2, gsmem, st>>>(nullptr, nd_i, ng, Bh, Bl, rB, nb, nbp, nkb,
+    mfma_contract2<1, 0, true><<<grid, kThreads2, gsmem, st>>>(nullptr, nd_i, ng, Bh, Bl, rB, nb, nbp, nkb,
                                                                    ntn, d_out, gen);
   }
   RGN_HIP(hipGetLastError());
@@ -833,6 +861,7 @@ void contraction_bf16(const SyncPlan& plan, const float* d_k, int ng, const floa
   RGN_REQUIRE(nd <= static_cast<size_t>(INT32_MAX) - kTM, RGN_ERR_RANGE, "too many distributions");
   const bool generated = d_index != nullptr;
   RGN_REQUIRE(!generated || ng >= 2, RGN_ERR_INVALID, "a generated distribution needs at least two grid points");
+  RGN_REQUIRE(!generated || ng <= kGenMaxNg, RGN_ERR_RANGE, "generated distributions take at most 2048 grid points");
   const int nb = plan.nbins;
   const int nbp = static_cast<int>(ceil_div(nb, kTN)) * kTN;
   const int nkb = static_cast<int>(ceil_div(ng, kKB));
@@ -840,24 +869,25 @@ void contraction_bf16(const SyncPlan& plan, const float* d_k, int ng, const floa
   __bf16* Bh = static_cast<__bf16*>(scratch(3 * sizeof(__bf16) * packed, 19));
   __bf16* Bm = Bh + packed;
   __bf16* Bl = Bm + packed;
-  pack_kernel_matrix_bf16<<<dim3(nbp / 256 + (nbp % 256 != 0), nkb), 256, 0, st>>>(d_k, ng, nb, nbp, Bh, Bm, Bl);
+  pack_kernel_matrix_bf16<<<static_cast<unsigned>(ceil_div(packed, 256)), 256, 0, st>>>(d_k, ng, nb, nbp, packed, Bh,
+                                                                                         Bm, Bl);
   RGN_HIP(hipGetLastError());
   stage_mark(st, "pack_kernel_matrix_bf16");
-  GenRows gen{};
-  if (generated) gen = prepare_gen_rows(d_gbeta, ng, d_index, d_cutoff, nd, st);
   if (nd == 0) return;
+  const GenRows gen{d_index, d_cutoff, d_gbeta};
   const int ntn = nbp / kTN;
   const unsigned grid = static_cast<unsigned>(ceil_div(nd, kTM2) * ntn);
   const int nd_i = static_cast<int>(nd);
+  const bool vec = ng % 4 == 0 && reinterpret_cast<uintptr_t>(d_f) % 16 == 0;
   if (generated) {
-    mfma_contract_bf16<false, true><<<grid, kThreads2, 2 * sizeof(double) * static_cast<size_t>(ng), st>>>(
+    mfma_contract_bf16<0, true><<<grid, kThreads2, 3 * sizeof(double) * static_cast<size_t>(ng), st>>>(
       nullptr, nd_i, ng, Bh, Bm, Bl, nb, nbp, nkb, ntn, d_out, gen);
-  } else if (ng % 4 == 0 && reinterpret_cast<uintptr_t>(d_f) % 16 == 0) {
-    mfma_contract_bf16<true, false><<<grid, kThreads2, 0, st>>>(d_f, nd_i, ng, Bh, Bm, Bl, nb, nbp, nkb, ntn, d_out,
-                                                                gen);
+  } else if (vec && ng % kKB == 0) {
+    mfma_contract_bf16<2, false><<<grid, kThreads2, 0, st>>>(d_f, nd_i, ng, Bh, Bm, Bl, nb, nbp, nkb, ntn, d_out, gen);
+  } else if (vec) {
+    mfma_contract_bf16<1, false><<<grid, kThreads2, 0, st>>>(d_f, nd_i, ng, Bh, Bm, Bl, nb, nbp, nkb, ntn, d_out, gen);
   } else {
-    mfma_contract_bf16<false, false><<<grid, kThreads2, 0, st>>>(d_f, nd_i, ng, Bh, Bm, Bl, nb, nbp, nkb, ntn, d_out,
-                                                                 gen);
+    mfma_contract_bf16<0, false><<<grid, kThreads2, 0, st>>>(d_f, nd_i, ng, Bh, Bm, Bl, nb, nbp, nkb, ntn, d_out, gen);
   }
   RGN_HIP(hipGetLastError());
   stage_mark(st, "mfma_contract_bf16");

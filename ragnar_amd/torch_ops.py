@@ -210,8 +210,9 @@ def sync_fromdist_batched(plan: SyncPlan, gbeta, f2d, dist_log: bool, precision:
 def sync_fromdist_batched_plaw(plan: SyncPlan, gbeta, index, cutoff, dist_log: bool, precision: int,
                                out=None) -> torch.Tensor:
     """Spectra of nd power laws with exponential cutoff, f_d(x) = x^index_d
-    exp(-x / cutoff_d) normalised by the trapezoid rule over the gbeta grid,
-    generated inside the split-FP16 kernel (precision RGN_PREC_F16X3 or _F16X1)."""
+    exp(-x / cutoff_d) normalised by the trapezoid rule over the gbeta grid
+    (2 to 2048 points), generated inside the matrix-core kernels (precision
+    RGN_PREC_F16X3, _F16X1 or _BF16X6)."""
     ng = gbeta.shape[0]
     nd = index.shape[0]
     pg = _vec(gbeta, "gbeta", ng)

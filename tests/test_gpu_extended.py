@@ -270,17 +270,25 @@ def test_adjoint_methods_agree_large(orc, table, weighted):
 
 
 @pytest.mark.parametrize("precision,tol", [(1, 1e-5), (2, 3e-3), (3, 1e-5)])
-def test_contraction_split_fp16_rows(orc, table, precision, tol):
+@pytest.mark.parametrize("ng,offset", [(500, 0), (512, 0), (512, 1)])
+def test_contraction_split_fp16_rows(orc, table, precision, tol, ng, offset):
     """Matrix-core contraction (split FP16: F16X3 = 1, F16X1 = 2; split BF16:
-    BF16X6 = 3) against the oracle's FP64 FromDist of each row, on sizes that
-    are not multiples of the tiles.  Bins above 1e-12 of a row's peak must be
-    within tol (F16X1, one fp16 product, only above 1e-6 of the peak)."""
-    nd, ng, nb = 300, 500, 200
+    BF16X6 = 3) against the oracle's FP64 FromDist of each row, with 300 rows
+    (not a multiple of the 64-row tile) and each way the kernels read f: 500
+    g (16-byte loads, the last g block partial), 512 g (whole 64-g blocks) and
+    512 g from a base address 4 bytes past a 16-byte boundary (scalar loads).
+    Bins above 1e-12 of a row's peak must be within tol (F16X1, one fp16
+    product, only above 1e-6 of the peak)."""
+    nd, nb = 300, 200
     f, _, _ = workloads.distributions(nd=nd, ng=ng, seed=7)
     g = orc.logspace(1, 1e6, ng)
     bins = orc.logspace(1e-2, 1e7, nb)
     plan = T.SyncPlan(table, bins)
-    S = T.sync_fromdist_batched(plan, dev(g), dev(f), True, precision).cpu().numpy().astype(np.float64)
+    F = torch.zeros(nd * ng + offset, device=DEV)
+    F = F[offset:].view(nd, ng)
+    F.copy_(dev(f))
+    assert (F.data_ptr() % 16 != 0) == bool(offset)
+    S = T.sync_fromdist_batched(plan, dev(g), F, True, precision).cpu().numpy().astype(np.float64)
     S32 = T.sync_fromdist_batched(plan, dev(g), dev(f), True, 0).cpu().numpy().astype(np.float64)
     worst = {1e-6: 0.0, 1e-9: 0.0, 1e-12: 0.0}
     worst32 = 0.0
@@ -379,3 +387,8 @@ def test_contraction_generated_rows(orc, table, precision):
     assert worst_l <= 2e-6
     with pytest.raises(RuntimeError):  # the FP32 GEMM takes rows as values only
         T.sync_fromdist_batched_plaw(plan, G, dev(p.astype(np.float32)), dev(gcut.astype(np.float32)), True, 0)
+    # the grid's terms are staged in LDS: at most 2048 points, refused before any launch
+    g_big = dev(orc.logspace(1, 1e6, 2049))
+    with pytest.raises(ValueError):
+        T.sync_fromdist_batched_plaw(plan, g_big, dev(p.astype(np.float32)), dev(gcut.astype(np.float32)), True,
+                                     precision)
