@@ -3,12 +3,14 @@
 // applied to every row, written as a GEMM.
 //
 // K[g, e] = e_e * (gbeta_g if the distribution bins are log spaced) *
-// F(e_e / eps_g), eps_g = e_at gbeta_g^2 / g_syn^2, is built on the GPU with the
-// same interpolation coordinate and segment table as the emission kernels
-// (one thread per entry).  The GEMM then runs on the matrix cores:
-//   RGN_PREC_FP32  rocBLAS SGEMM (a plain library GEMM, FP32 throughout);
-//   RGN_PREC_F16X3 / RGN_PREC_F16X1: split-FP16 MFMA kernel (contraction_f16.hip),
-//   FP32 accumulation with per-block power-of-two scaling.
+// F(e_e / eps_g), eps_g = e_at gbeta_g^2 / g_syn^2, is formed on the GPU with
+// the same interpolation coordinate and segment table as the emission kernels
+// (dist_kernel_value).  The GEMM then runs on the matrix cores:
+//   RGN_PREC_FP32  rocBLAS SGEMM on K built in full (kernel_matrix; a plain
+//                  library GEMM, FP32 throughout);
+//   RGN_PREC_F16X3 / _F16X1 / _BF16X6: hand-written MFMA kernels
+//                  (contraction_f16.hip), whose packing kernels form K's
+//                  entries themselves, FP32 accumulation.
 #include <rocblas/rocblas.h>
 
 #include <mutex>
@@ -19,13 +21,13 @@
 namespace rgn {
 
 // Split-FP16 MFMA form, contraction_f16.hip.
-void contraction_f16(const SyncPlan& plan, const float* d_k, int ng, const float* d_f, size_t nd, int nprod,
-                     float* d_out, hipStream_t st);
+void contraction_f16(const SyncPlan& plan, const float* d_gbeta, int ng, int dist_log, const float* d_f, size_t nd,
+                     int nprod, float* d_out, hipStream_t st);
 // Split-BF16 form: rows from d_f, or generated when d_index is not null.
-void contraction_bf16(const SyncPlan& plan, const float* d_k, int ng, const float* d_f, const float* d_gbeta,
+void contraction_bf16(const SyncPlan& plan, const float* d_gbeta, int ng, int dist_log, const float* d_f,
                       const float* d_index, const float* d_cutoff, size_t nd, float* d_out, hipStream_t st);
 // The same with the rows generated from (index, cutoff) inside the kernel.
-void contraction_f16_plaw(const SyncPlan& plan, const float* d_k, int ng, const float* d_gbeta, const float* d_index,
+void contraction_f16_plaw(const SyncPlan& plan, const float* d_gbeta, int ng, int dist_log, const float* d_index,
                           const float* d_cutoff, size_t nd, int nprod, float* d_out, hipStream_t st);
 
 namespace {
@@ -35,17 +37,7 @@ __global__ void __launch_bounds__(256) kernel_matrix(PlanDev P, const float* __r
   const int e = blockIdx.x * 256 + threadIdx.x;
   const int g = blockIdx.y;
   if (e >= P.nbins) return;
-  const float gb = gbeta[g];
-  const float eps = P.em.e_at * gb * gb / (P.em.g_syn * P.em.g_syn);  // synchrotron.hpp:78-79
-  const Source s = make_source(dist_log ? gb : 1.0f, eps, P);
-  float v = 0.0f;
-  if (s.W != 0.0f) {
-    const float2 cs_e = P.bincs[e];
-    const float u = fmaf(cs_e.x, cs_e.y, s.vhi) + s.vlo;
-    const float2 cs = P.table[cvt_sat_u32(u)];
-    v = P.bine[e] * (s.W * fmaf(u, cs.y, cs.x));
-  }
-  K[static_cast<size_t>(g) * P.nbins + e] = v;
+  K[static_cast<size_t>(g) * P.nbins + e] = dist_kernel_value(dist_source(gbeta[g], dist_log, P), e, P);
 }
 
 std::mutex g_blas_mutex;
@@ -83,10 +75,10 @@ extern "C" int rgn_sync_fromdist_batched(const rgn_sync_plan* plan, const float*
     const SyncPlan& p = plan->p;
     hipStream_t st = as_stream(stream);
     ScratchUse order(st);
-    float* d_k = static_cast<float*>(scratch(sizeof(float) * static_cast<size_t>(ng) * p.nbins, 5));
-    build_kernel_matrix(p, d_gbeta, ng, dist_log, d_k, st);
-    if (nd == 0) return;
     if (precision == RGN_PREC_FP32) {
+      float* d_k = static_cast<float*>(scratch(sizeof(float) * static_cast<size_t>(ng) * p.nbins, 5));
+      build_kernel_matrix(p, d_gbeta, ng, dist_log, d_k, st);
+      if (nd == 0) return;
       std::lock_guard<std::mutex> lk(g_blas_mutex);
       rocblas_handle h = blas();
       RGN_REQUIRE(rocblas_set_stream(h, st) == rocblas_status_success, RGN_ERR_HIP,
@@ -98,9 +90,9 @@ extern "C" int rgn_sync_fromdist_batched(const rgn_sync_plan* plan, const float*
                                              p.nbins, d_f, ng, &zero, d_out, p.nbins);
       RGN_REQUIRE(s == rocblas_status_success, RGN_ERR_HIP, "rocblas_sgemm failed");
     } else if (precision == RGN_PREC_F16X3 || precision == RGN_PREC_F16X1) {
-      contraction_f16(p, d_k, ng, d_f, nd, precision == RGN_PREC_F16X3 ? 3 : 1, d_out, st);
+      contraction_f16(p, d_gbeta, ng, dist_log, d_f, nd, precision == RGN_PREC_F16X3 ? 3 : 1, d_out, st);
     } else if (precision == RGN_PREC_BF16X6) {
-      contraction_bf16(p, d_k, ng, d_f, d_gbeta, nullptr, nullptr, nd, d_out, st);
+      contraction_bf16(p, d_gbeta, ng, dist_log, d_f, nullptr, nullptr, nd, d_out, st);
     } else {
       throw Error(RGN_ERR_INVALID, "unknown precision mode");
     }
@@ -121,13 +113,11 @@ extern "C" int rgn_sync_fromdist_batched_plaw(const rgn_sync_plan* plan, const f
     const SyncPlan& p = plan->p;
     hipStream_t st = as_stream(stream);
     ScratchUse order(st);
-    float* d_k = static_cast<float*>(scratch(sizeof(float) * static_cast<size_t>(ng) * p.nbins, 5));
-    build_kernel_matrix(p, d_gbeta, ng, dist_log, d_k, st);
     if (precision == RGN_PREC_BF16X6) {
-      contraction_bf16(p, d_k, ng, nullptr, d_gbeta, d_index, d_cutoff, nd, d_out, st);
+      contraction_bf16(p, d_gbeta, ng, dist_log, nullptr, d_index, d_cutoff, nd, d_out, st);
     } else {
-      contraction_f16_plaw(p, d_k, ng, d_gbeta, d_index, d_cutoff, nd, precision == RGN_PREC_F16X3 ? 3 : 1, d_out,
-                           st);
+      contraction_f16_plaw(p, d_gbeta, ng, dist_log, d_index, d_cutoff, nd, precision == RGN_PREC_F16X3 ? 3 : 1,
+                           d_out, st);
     }
   });
 }

@@ -65,28 +65,44 @@ __device__ __forceinline__ size_t packed_index(int kb, int c, int i, int nbp, in
                  : (static_cast<size_t>(kb) * nbp + c) * kKB + i;
 }
 
-// One block per (16 columns, g block): the 64 x 16 tile of K is read into
-// LDS (each row segment 64 contiguous bytes), each column's block maximum sets
-// its exponent (16 partial maxima per column, then their maximum), and the
-// split halves are written in packed order, consecutive threads to
-// consecutive addresses.
+// The 64 x 16 tile of K for g block kb and columns c0 .. c0 + 15, formed in
+// LDS: the block's 64 sources first (one FP64 log2 each), then the entries
+// (dist_kernel_value, the same float operations as kernel_matrix, so K is
+// the same bit for bit).  Entries past ng or nb are 0.
 constexpr int kPackCols = 16;
 
-__global__ void __launch_bounds__(256) pack_kernel_matrix(const float* __restrict__ K, int ng, int nb, int nbp,
-                                                          _Float16* __restrict__ Bh, _Float16* __restrict__ Bl,
-                                                          int* __restrict__ sB, float* __restrict__ rB,
-                                                          int ksmajor) {
+__device__ __forceinline__ void form_k_tile(const PlanDev& P, const float* __restrict__ gbeta, int ng, int dist_log,
+                                            int kb, int c0, float (&tile)[kKB][kPackCols + 1], Source (&src)[kKB]) {
+  const int t = threadIdx.x;
+  if (t < kKB) {
+    const int g = kb * kKB + t;
+    src[t] = g < ng ? dist_source(gbeta[g], dist_log, P) : Source{0.0f, P.v_safe, 0.0f};
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kKB * kPackCols / 256; ++r) {
+    const int idx = r * 256 + t, i = idx / kPackCols, c = idx % kPackCols;
+    tile[i][c] = c0 + c < P.nbins ? dist_kernel_value(src[i], c0 + c, P) : 0.0f;
+  }
+  __syncthreads();
+}
+
+// Split K into the packed hi/lo halves.  One block per (16 columns, g
+// block): the tile is formed in LDS, each column's block maximum sets its
+// exponent (16 partial maxima per column, then their maximum), and the split
+// halves are written in packed order, consecutive threads to consecutive
+// addresses.
+__global__ void __launch_bounds__(256) pack_kernel_matrix(PlanDev P, const float* __restrict__ gbeta, int ng,
+                                                          int dist_log, int nbp, _Float16* __restrict__ Bh,
+                                                          _Float16* __restrict__ Bl, int* __restrict__ sB,
+                                                          float* __restrict__ rB, int ksmajor) {
   __shared__ float tile[kKB][kPackCols + 1];
+  __shared__ Source src[kKB];
   __shared__ float part[16][kPackCols];
   __shared__ int sexp[kPackCols];
   const int c0 = blockIdx.x * kPackCols, kb = blockIdx.y;
   const int t = threadIdx.x;
-#pragma unroll
-  for (int r = 0; r < kKB * kPackCols / 256; ++r) {
-    const int idx = r * 256 + t, i = idx / kPackCols, c = idx % kPackCols, g = kb * kKB + i;
-    tile[i][c] = (c0 + c < nb && g < ng) ? K[static_cast<size_t>(g) * nb + c0 + c] : 0.0f;
-  }
-  __syncthreads();
+  form_k_tile(P, gbeta, ng, dist_log, kb, c0, tile, src);
   {
     const int c = t % kPackCols, q = t / kPackCols;  // rows 4q .. 4q + 3
     float m = 0.0f;
@@ -345,7 +361,8 @@ __device__ __forceinline__ void stage_grid_terms(const float* __restrict__ grid,
 //   LOAD 2  16-byte loads of whole blocks (and ng % 64 == 0): rows past nd
 //           read row nd - 1, whose results are computed but never stored;
 //   GEN     generated (GenRows; grid terms staged in LDS by the kernel),
-//           rows past nd generate row nd - 1.
+//           rows past nd generate row nd - 1; with LOAD 2 (ng % 64 == 0) no
+//           element is checked against ng.
 // load(kb, ra, count): block kb into ra; with count (GEN), the block's
 // values also go into the row's norm (each block is counted once, though the
 // kernels' branch-free loops load the last block more than once).
@@ -379,7 +396,7 @@ struct RowBlocks {
       double s = 0.0;
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
-        const bool in = g0 + i < ng;
+        const bool in = LOAD == 2 || g0 + i < ng;
         ra[i] = in ? gen_value(gp, gc, go, gL[g0 + i], gL[ng + g0 + i]) : 0.0f;
         s = fma(static_cast<double>(ra[i]), in ? gL[2 * ng + g0 + i] : 0.0, s);
       }
@@ -610,25 +627,27 @@ __device__ __forceinline__ void split3(float x, __bf16& h, __bf16& m, __bf16& l)
   l = static_cast<__bf16>(r1 - static_cast<float>(m));  // exact difference
 }
 
-// K[g][e] -> B{h,m,l} in the step-major packed layout: one thread per packed
-// element, in output order.
-__global__ void __launch_bounds__(256) pack_kernel_matrix_bf16(const float* __restrict__ K, int ng, int nb, int nbp,
-                                                               size_t packed, __bf16* __restrict__ Bh,
+// K -> B{h,m,l} in the step-major packed layout.  One block per (16 columns,
+// g block): the tile is formed in LDS (form_k_tile), split and written in
+// packed order, consecutive threads to consecutive addresses.
+__global__ void __launch_bounds__(256) pack_kernel_matrix_bf16(PlanDev P, const float* __restrict__ gbeta, int ng,
+                                                               int dist_log, int nbp, __bf16* __restrict__ Bh,
                                                                __bf16* __restrict__ Bm, __bf16* __restrict__ Bl) {
-  const size_t at = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
-  if (at >= packed) return;
-  // at = ((kb * 4 + ks) * nbp + c) * 16 + i16
-  const int i16 = static_cast<int>(at % 16);
-  const size_t r = at / 16;
-  const int c = static_cast<int>(r % nbp);
-  const int step = static_cast<int>(r / nbp);
-  const int g = step * 16 + i16;
-  const float x = (c < nb && g < ng) ? K[static_cast<size_t>(g) * nb + c] : 0.0f;
-  __bf16 h, m, l;
-  split3(x, h, m, l);
-  Bh[at] = h;
-  Bm[at] = m;
-  Bl[at] = l;
+  __shared__ float tile[kKB][kPackCols + 1];
+  __shared__ Source src[kKB];
+  const int c0 = blockIdx.x * kPackCols, kb = blockIdx.y;
+  form_k_tile(P, gbeta, ng, dist_log, kb, c0, tile, src);
+#pragma unroll
+  for (int r = 0; r < kKB * kPackCols / 256; ++r) {
+    const int idx = r * 256 + threadIdx.x;
+    const int c = (idx / 16) % kPackCols, i = (idx / (16 * kPackCols)) * 16 + idx % 16;
+    __bf16 h, m, l;
+    split3(tile[i][c], h, m, l);
+    const size_t at = packed_index(kb, c0 + c, i, nbp, 1);
+    Bh[at] = h;
+    Bm[at] = m;
+    Bl[at] = l;
+  }
 }
 
 template <int LOAD, bool GEN>
@@ -768,10 +787,11 @@ __global__ void __launch_bounds__(kThreads2, 2) mfma_contract_bf16(const float* 
 
 }  // namespace
 
-void contraction_f16(const SyncPlan& plan, const float* d_k, int ng, const float* d_f, size_t nd, int nprod,
-                     float* d_out, hipStream_t st) {
+void contraction_f16(const SyncPlan& plan, const float* d_gbeta, int ng, int dist_log, const float* d_f, size_t nd,
+                     int nprod, float* d_out, hipStream_t st) {
   RGN_REQUIRE(nprod == 1 || nprod == 3, RGN_ERR_INVALID, "split-FP16 products must be 1 or 3");
   RGN_REQUIRE(nd <= static_cast<size_t>(INT32_MAX) - kTM, RGN_ERR_RANGE, "too many distributions");
+  if (nd == 0) return;
   const int nb = plan.nbins;
   const int nbp = static_cast<int>(ceil_div(nb, kTN)) * kTN;
   const int nkb = static_cast<int>(ceil_div(ng, kKB));
@@ -782,7 +802,8 @@ void contraction_f16(const SyncPlan& plan, const float* d_k, int ng, const float
   int* sB = reinterpret_cast<int*>(Bl + packed);
   float* rB = reinterpret_cast<float*>(sB + static_cast<size_t>(nkb) * nbp);
   const bool form1 = env_int("RAGNAR_CONTRACT_FORM", 2) == 1;
-  pack_kernel_matrix<<<dim3(nbp / kPackCols, nkb), 256, 0, st>>>(d_k, ng, nb, nbp, Bh, Bl, sB, rB, form1 ? 0 : 1);
+  pack_kernel_matrix<<<dim3(nbp / kPackCols, nkb), 256, 0, st>>>(device_view(plan), d_gbeta, ng, dist_log, nbp, Bh,
+                                                                  Bl, sB, rB, form1 ? 0 : 1);
   RGN_HIP(hipGetLastError());
   stage_mark(st, "pack_kernel_matrix");
   // 16-byte loads of f when every row starts 16-byte aligned
@@ -819,12 +840,13 @@ void contraction_f16(const SyncPlan& plan, const float* d_k, int ng, const float
   stage_mark(st, "mfma_contract");
 }
 
-void contraction_f16_plaw(const SyncPlan& plan, const float* d_k, int ng, const float* d_gbeta, const float* d_index,
+void contraction_f16_plaw(const SyncPlan& plan, const float* d_gbeta, int ng, int dist_log, const float* d_index,
                           const float* d_cutoff, size_t nd, int nprod, float* d_out, hipStream_t st) {
   RGN_REQUIRE(nprod == 1 || nprod == 3, RGN_ERR_INVALID, "split-FP16 products must be 1 or 3");
   RGN_REQUIRE(ng >= 2, RGN_ERR_INVALID, "a generated distribution needs at least two grid points");
   RGN_REQUIRE(ng <= kGenMaxNg, RGN_ERR_RANGE, "generated distributions take at most 2048 grid points");
   RGN_REQUIRE(nd <= static_cast<size_t>(INT32_MAX) - kTM, RGN_ERR_RANGE, "too many distributions");
+  if (nd == 0) return;
   const int nb = plan.nbins;
   const int nbp = static_cast<int>(ceil_div(nb, kTN)) * kTN;
   const int nkb = static_cast<int>(ceil_div(ng, kKB));
@@ -834,34 +856,38 @@ void contraction_f16_plaw(const SyncPlan& plan, const float* d_k, int ng, const 
   _Float16* Bl = Bh + packed;
   int* sB = reinterpret_cast<int*>(Bl + packed);
   float* rB = reinterpret_cast<float*>(sB + static_cast<size_t>(nkb) * nbp);
-  pack_kernel_matrix<<<dim3(nbp / kPackCols, nkb), 256, 0, st>>>(d_k, ng, nb, nbp, Bh, Bl, sB, rB, 1);
+  pack_kernel_matrix<<<dim3(nbp / kPackCols, nkb), 256, 0, st>>>(device_view(plan), d_gbeta, ng, dist_log, nbp, Bh,
+                                                                  Bl, sB, rB, 1);
   RGN_HIP(hipGetLastError());
   stage_mark(st, "pack_kernel_matrix");
-  if (nd == 0) return;
   const GenRows gen{d_index, d_cutoff, d_gbeta};
   const size_t gsmem = 3 * sizeof(double) * static_cast<size_t>(ng);
   const int ntn = nbp / kTN;
   const unsigned grid = static_cast<unsigned>(ceil_div(nd, kTM2) * ntn);
   const int nd_i = static_cast<int>(nd);
+#define RGN_CONTRACT_GEN(P_, L_)                                                                            \
+  mfma_contract2<P_, L_, true><<<grid, kThreads2, gsmem, st>>>(nullptr, nd_i, ng, Bh, Bl, rB, nb, nbp, nkb, ntn, \
+                                                              d_out, gen)
+  const bool full = ng % kKB == 0;
   if (nprod == 3) {
-    mfma_contract2<3, 0, true><<<grid, kThreads2, gsmem, st>>>(nullptr, nd_i, ng, Bh, Bl, rB, nb, nbp, nkb,
-                                                                   ntn, d_out, gen);
+    if (full) RGN_CONTRACT_GEN(3, 2); else RGN_CONTRACT_GEN(3, 0);
   } else {
-    mfma_contract2<1, 0, true><<<grid, kThreads2, gsmem, st>>>(nullptr, nd_i, ng, Bh, Bl, rB, nb, nbp, nkb,
-                                                                   ntn, d_out, gen);
+    if (full) RGN_CONTRACT_GEN(1, 2); else RGN_CONTRACT_GEN(1, 0);
   }
+#undef RGN_CONTRACT_GEN
   RGN_HIP(hipGetLastError());
   stage_mark(st, "mfma_contract");
 }
 
 // Split-BF16 form.  Rows come from d_f (nd x ng, row-major) or, when d_index is
 // not null, are generated from (index, cutoff) over the grid d_gbeta.
-void contraction_bf16(const SyncPlan& plan, const float* d_k, int ng, const float* d_f, const float* d_gbeta,
+void contraction_bf16(const SyncPlan& plan, const float* d_gbeta, int ng, int dist_log, const float* d_f,
                       const float* d_index, const float* d_cutoff, size_t nd, float* d_out, hipStream_t st) {
   RGN_REQUIRE(nd <= static_cast<size_t>(INT32_MAX) - kTM, RGN_ERR_RANGE, "too many distributions");
   const bool generated = d_index != nullptr;
   RGN_REQUIRE(!generated || ng >= 2, RGN_ERR_INVALID, "a generated distribution needs at least two grid points");
   RGN_REQUIRE(!generated || ng <= kGenMaxNg, RGN_ERR_RANGE, "generated distributions take at most 2048 grid points");
+  if (nd == 0) return;
   const int nb = plan.nbins;
   const int nbp = static_cast<int>(ceil_div(nb, kTN)) * kTN;
   const int nkb = static_cast<int>(ceil_div(ng, kKB));
@@ -869,19 +895,22 @@ void contraction_bf16(const SyncPlan& plan, const float* d_k, int ng, const floa
   __bf16* Bh = static_cast<__bf16*>(scratch(3 * sizeof(__bf16) * packed, 19));
   __bf16* Bm = Bh + packed;
   __bf16* Bl = Bm + packed;
-  pack_kernel_matrix_bf16<<<static_cast<unsigned>(ceil_div(packed, 256)), 256, 0, st>>>(d_k, ng, nb, nbp, packed, Bh,
-                                                                                         Bm, Bl);
+  pack_kernel_matrix_bf16<<<dim3(nbp / kPackCols, nkb), 256, 0, st>>>(device_view(plan), d_gbeta, ng, dist_log, nbp,
+                                                                       Bh, Bm, Bl);
   RGN_HIP(hipGetLastError());
   stage_mark(st, "pack_kernel_matrix_bf16");
-  if (nd == 0) return;
   const GenRows gen{d_index, d_cutoff, d_gbeta};
   const int ntn = nbp / kTN;
   const unsigned grid = static_cast<unsigned>(ceil_div(nd, kTM2) * ntn);
   const int nd_i = static_cast<int>(nd);
   const bool vec = ng % 4 == 0 && reinterpret_cast<uintptr_t>(d_f) % 16 == 0;
-  if (generated) {
-    mfma_contract_bf16<0, true><<<grid, kThreads2, 3 * sizeof(double) * static_cast<size_t>(ng), st>>>(
-      nullptr, nd_i, ng, Bh, Bm, Bl, nb, nbp, nkb, ntn, d_out, gen);
+  const size_t gsmem = 3 * sizeof(double) * static_cast<size_t>(ng);
+  if (generated && ng % kKB == 0) {
+    mfma_contract_bf16<2, true><<<grid, kThreads2, gsmem, st>>>(nullptr, nd_i, ng, Bh, Bm, Bl, nb, nbp, nkb, ntn,
+                                                                d_out, gen);
+  } else if (generated) {
+    mfma_contract_bf16<0, true><<<grid, kThreads2, gsmem, st>>>(nullptr, nd_i, ng, Bh, Bm, Bl, nb, nbp, nkb, ntn,
+                                                                d_out, gen);
   } else if (vec && ng % kKB == 0) {
     mfma_contract_bf16<2, false><<<grid, kThreads2, 0, st>>>(d_f, nd_i, ng, Bh, Bm, Bl, nb, nbp, nkb, ntn, d_out, gen);
   } else if (vec) {

@@ -282,6 +282,23 @@ __device__ __forceinline__ Source make_source(float W, float eps, const PlanDev&
   return s;
 }
 
+// One entry of the distribution kernel matrix K[g, e] (KernelFromDist,
+// synchrotron.hpp:72-95): the source of gamma-beta bin g, eps = e_at gb^2 /
+// g_syn^2 and W = gb for log-spaced distribution bins (synchrotron.hpp:78-79),
+// evaluated at photon bin e as the emission kernels evaluate a source.
+__device__ __forceinline__ Source dist_source(float gb, int dist_log, const PlanDev& P) {
+  const float eps = P.em.e_at * gb * gb / (P.em.g_syn * P.em.g_syn);
+  return make_source(dist_log ? gb : 1.0f, eps, P);
+}
+
+__device__ __forceinline__ float dist_kernel_value(const Source& s, int e, const PlanDev& P) {
+  if (s.W == 0.0f) return 0.0f;
+  const float2 cs_e = P.bincs[e];
+  const float u = fmaf(cs_e.x, cs_e.y, s.vhi) + s.vlo;
+  const float2 cs = P.table[cvt_sat_u32(u)];
+  return P.bine[e] * (s.W * fmaf(u, cs.y, cs.x));
+}
+
 __device__ __forceinline__ float load_or_zero(const float* p, size_t i) {
   return p ? p[i] : 0.0f;
 }

@@ -357,14 +357,16 @@ def test_fit_recovers_index_and_cutoff(table):
 
 
 @pytest.mark.parametrize("precision", [1, 3])
-def test_contraction_generated_rows(orc, table, precision):
+@pytest.mark.parametrize("ng", [500, 512])
+def test_contraction_generated_rows(orc, table, precision, ng):
     """Rows generated inside the kernel from (index, cutoff) against the same
     rows given as values (within 2e-6: the generated values differ from the
     float rows by about an ulp) and against the oracle's FP64 FromDist of each
     row as the kernel defines it (x^p exp(-x / xc) normalised by the
     trapezoid rule, p and xc as floats) at 1e-5 over bins above 1e-12 of the
-    peak; ragged sizes (300 rows, 500 g, 200 bins)."""
-    nd, ng, nb = 300, 500, 200
+    peak; 300 rows and 200 bins (partial tiles), 500 g (the last g block
+    partial) and 512 g (whole blocks, no per-element grid checks)."""
+    nd, nb = 300, 200
     f, p, gcut = workloads.distributions(nd=nd, ng=ng, seed=17)
     g = orc.logspace(1, 1e6, ng)
     bins = orc.logspace(1e-2, 1e7, nb)
