@@ -256,7 +256,7 @@ def summarize_compute(trace_dir, pmc_dir):
           "forward at 2^24 particles x 256 bins, the batched contraction at 65536 x 512 x 512",
           "(split FP16 `mfma_contract2<3, 2, false>`: three products, f read as whole 64-g blocks,",
           "`<1, 2, false>`: one product; split BF16 `mfma_contract_bf16<2, false>`: six products;",
-          "`mfma_contract2<3, 0, true>` and `mfma_contract_bf16<0, true>`: rows generated in the",
+          "`mfma_contract2<3, 2, true>` and `mfma_contract_bf16<2, true>`: rows generated in the",
           "kernel) and the inverse-Compton",
           "spectrum at 1000 gamma x 1000 soft-photon x 1000 output bins (`ic_kernel`).  The",
           "matrix rate counts every product pass (2 x 65536 x 512 x 512 flop each) over the traced",
