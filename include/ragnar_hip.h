@@ -185,12 +185,18 @@ int rgn_sync_fromdist(const rgn_sync_plan* plan, const float* d_gbeta, const flo
                       int ng, int dist_log, float* d_out, rgn_stream_t stream);
 
 /* Batched contraction S[d, e] = sum_g f[d, g] K[g, e] for nd distributions
- * sharing one gbeta grid (row-major d_f[nd * ng], d_out[nd * nbins]).
- * K[g, e] is KernelFromDist's term (synchrotron.hpp:78-93) built on the GPU.
+ * sharing one gbeta grid (row-major d_f[nd * ng], d_out[nd * nbins]; both
+ * may be NULL when nd == 0).  K[g, e] is KernelFromDist's term
+ * (synchrotron.hpp:78-93) formed on the GPU.
  * precision: RGN_PREC_FP32 (the library GEMM: rocBLAS SGEMM, FP32
  * throughout), RGN_PREC_F16X3 (the hand-written split-FP16 MFMA kernel: each
  * operand split into two fp16 parts with power-of-two scales per block of 64
- * g, three products, FP32 accumulation; more accurate than the FP32 GEMM),
+ * g, three products, FP32 accumulation; every element within 2^18 of its
+ * block's largest keeps 22 bits, so the result holds ~1e-6 when each block of
+ * 64 grid points spans about a decade of gbeta or less (512 log points over 6
+ * decades: 4.5e-7 from FP64, better than the FP32 GEMM; 64 points over 4
+ * decades: 0.19 in the highest bin), RGN_PREC_BF16X6 below has no such
+ * condition),
  * RGN_PREC_F16X1 (one FP16 product; reported, not parity), RGN_PREC_BF16X6
  * (the hand-written split-BF16 MFMA kernel: each operand split exactly into
  * three bf16 parts, which need no scaling, six products, FP32 accumulation;
@@ -207,7 +213,7 @@ int rgn_sync_fromdist_batched(const rgn_sync_plan* plan, const float* d_gbeta, i
  * row over the grid (sum_g (f_g + f_g+1) (x_g+1 - x_g) / 2).  The rows and
  * their norms are generated inside the matrix-core kernels (FP64 exponent,
  * float values, FP64 norm), so the call reads 8 bytes per distribution
- * instead of 4 ng.  precision: RGN_PREC_F16X3, RGN_PREC_F16X1 or
+ * instead of 4 ng (d_index, d_cutoff and d_out may be NULL when nd == 0).  precision: RGN_PREC_F16X3, RGN_PREC_F16X1 or
  * RGN_PREC_BF16X6 (RGN_ERR_UNSUPPORTED otherwise); 2 <= ng <= 2048
  * (RGN_ERR_INVALID below, RGN_ERR_RANGE above: the grid's FP64 terms are
  * staged in LDS).

@@ -70,7 +70,7 @@ extern "C" int rgn_sync_fromdist_batched(const rgn_sync_plan* plan, const float*
                                          float* d_out, rgn_stream_t stream) {
   return guarded([&] {
     RGN_REQUIRE(plan != nullptr, RGN_ERR_INVALID, "null plan");
-    RGN_REQUIRE(d_gbeta && d_f && d_out, RGN_ERR_INVALID, "null argument");
+    RGN_REQUIRE(d_gbeta && (nd == 0 || (d_f && d_out)), RGN_ERR_INVALID, "null argument");
     RGN_REQUIRE(ng > 0, RGN_ERR_INVALID, "empty distribution grid");
     const SyncPlan& p = plan->p;
     hipStream_t st = as_stream(stream);
@@ -104,7 +104,7 @@ extern "C" int rgn_sync_fromdist_batched_plaw(const rgn_sync_plan* plan, const f
                                               size_t nd, int precision, float* d_out, rgn_stream_t stream) {
   return guarded([&] {
     RGN_REQUIRE(plan != nullptr, RGN_ERR_INVALID, "null plan");
-    RGN_REQUIRE(d_gbeta && d_out && (nd == 0 || (d_index && d_cutoff)), RGN_ERR_INVALID, "null argument");
+    RGN_REQUIRE(d_gbeta && (nd == 0 || (d_out && d_index && d_cutoff)), RGN_ERR_INVALID, "null argument");
     RGN_REQUIRE(ng >= 2, RGN_ERR_INVALID, "a generated distribution needs at least two grid points");
     RGN_REQUIRE(ng <= 2048, RGN_ERR_RANGE, "generated distributions take at most 2048 grid points");
     RGN_REQUIRE(precision == RGN_PREC_F16X3 || precision == RGN_PREC_F16X1 || precision == RGN_PREC_BF16X6,

@@ -394,3 +394,53 @@ def test_contraction_generated_rows(orc, table, precision, ng):
     with pytest.raises(ValueError):
         T.sync_fromdist_batched_plaw(plan, g_big, dev(p.astype(np.float32)), dev(gcut.astype(np.float32)), True,
                                      precision)
+
+
+@pytest.mark.parametrize("precision", [1, 2, 3])
+@pytest.mark.parametrize("nd,ng,nb", [(1, 2, 1), (1, 64, 257), (65, 63, 1), (65, 65, 257), (3, 128, 300)])
+def test_contraction_small_and_ragged(orc, table, precision, nd, ng, nb):
+    """The matrix-core forms at sizes around their tiles (64 rows, 64 g per
+    block, 256 bins): one row, one bin, a single partial g block, a block
+    and one g, 257 and 300 bins (a second, nearly empty column tile), rows
+    given and generated, each row against the oracle's FP64 FromDist (1e-5 over
+    bins above 1e-12 of the row's peak; F16X1, one fp16 product, 3e-3 above
+    1e-6).  The split-FP16 forms get a grid of 0.7 decades per 64 points (their
+    stated condition); split BF16, which has none, 4 decades over the grid."""
+    rng = np.random.default_rng(nd * 1000 + ng * 10 + nb)
+    g = orc.logspace(1, 10.0 ** (4.0 if precision == 3 else 0.7 * ng / 64), ng)
+    bins = orc.logspace(1e-2, 1e7, nb)
+    plan = T.SyncPlan(table, bins)
+    p = rng.uniform(-3.5, -1.5, nd).astype(np.float32)
+    gcut = (10.0 ** rng.uniform(1.0, 3.5, nd)).astype(np.float32)
+    gd = g.astype(np.float64)
+    f = np.empty((nd, ng), np.float32)
+    for d in range(nd):
+        v = np.exp(np.float64(p[d]) * np.log(gd) - gd / np.float64(gcut[d]))
+        f[d] = (v / np.trapezoid(v, gd) if ng > 1 else v).astype(np.float32)
+    G = dev(g)
+    outs = {"given": T.sync_fromdist_batched(plan, G, dev(f), True, precision).double().cpu().numpy()}
+    if ng >= 2:
+        outs["generated"] = T.sync_fromdist_batched_plaw(plan, G, dev(p), dev(gcut), True,
+                                                         precision).double().cpu().numpy()
+    tol, thr = (3e-3, 1e-6) if precision == 2 else (1e-5, 1e-12)
+    for name, S in outs.items():
+        assert S.shape == (nd, nb) and np.all(np.isfinite(S)), name
+        for d in range(nd):
+            _, f64 = orc.sync_fromdist(g, f[d], True, bins, mode=orc.FP64, want_fp64=True)
+            live = f64 > thr * f64.max() if f64.max() > 0 else np.zeros(nb, bool)
+            if live.any():
+                err = float(np.max(np.abs(S[d][live] / f64[live] - 1)))
+                assert err <= tol, (name, d, err)
+            assert np.all(np.abs(S[d][f64 == 0]) <= 1e-10 * max(f64.max(), 1e-300)), name
+
+
+@pytest.mark.parametrize("precision", [1, 3])
+def test_contraction_no_rows(table, precision):
+    """nd = 0: nothing to compute, no launch, no error."""
+    g = dev(np.logspace(0, 3, 64).astype(np.float32))
+    plan = T.SyncPlan(table, np.logspace(-2, 7, 64).astype(np.float32))
+    S = T.sync_fromdist_batched(plan, g, torch.empty(0, 64, device=DEV), True, precision)
+    Sg = T.sync_fromdist_batched_plaw(plan, g, torch.empty(0, device=DEV), torch.empty(0, device=DEV), True,
+                                      precision)
+    torch.cuda.synchronize()
+    assert S.shape == (0, 64) and Sg.shape == (0, 64)
