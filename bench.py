@@ -786,8 +786,9 @@ def main():
 
     if args.summarize:
         if args.method == "compute":
-            summarize(*args.summarize, "compute", tag="compute_kernels",
-                      title="compute-bound kernels (bench.py --profile-compute)", source="--profile-compute")
+            summarize(*args.summarize, "compute", tag="compute_bound",
+                      title="forward pair by pair at 2^24 x 256, contraction at 65536 x 512 x 512, "
+                            "inverse Compton at 1000^3 (bench.py --profile-compute)", source="--profile-compute")
         else:
             summarize(*args.summarize, args.method)
         return
