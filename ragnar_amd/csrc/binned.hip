@@ -22,9 +22,10 @@
 //   scan      per bucket, its non-empty runs compacted in block order with
 //             their offsets within the bucket;
 //   subhist   each bucket is cut into work items of up to `item` sources (at
-//             least a bucket block's chunk; 16384 unless RAGNAR_SUBHIST_ITEM); each
-//             item walks its runs and reduces the sources into M + 1
-//             sub-cells in LDS (ds_add_f64);
+//             least a bucket block's chunk; sized so that the items run as one
+//             round of resident blocks, or RAGNAR_SUBHIST_ITEM); each item
+//             walks its runs and reduces the sources into M + 1 sub-cells in
+//             LDS (ds_add_f64);
 //   prefix    per bucket: the items in order, then prefix sums over sub-cells;
 //   evaluate  one block per photon bin, fixed-order reduction over k.
 // Adjoint (a gather): dL/dS is folded into a table U[k][j] = (sum_e g'_e C,
@@ -152,8 +153,7 @@ constexpr int kBucketThreads = 1024;       // threads of a bucket block (two per
 constexpr int kPer = 8;                    // particles per thread in the bucket pass
 constexpr int kChunk = kBucketThreads * kPer;  // particles per bucket block
 static_assert(kChunk <= 32768, "run descriptors pack start and length in 16 bits each");
-constexpr unsigned kItem = 16384;          // sources per sub-histogram work item (default)
-static_assert(kChunk <= kItem, "a run (at most one chunk) must hold at most one work item's first source");
+constexpr size_t kLdsPerCU = 160 * 1024;   // LDS of a gfx950 CU
 constexpr int kSubThreads = 512;           // threads of a sub-histogram work item
 constexpr int kWin = 2048;                 // runs per LDS window of a work item
 constexpr int kPrefThreads = 768;          // 3 item groups x 256 sub-cells per prefix block (two blocks per CU)
@@ -899,8 +899,18 @@ void sync_forward_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, 
   const int G = static_cast<int>(ceil_div(n, kChunk));
   const size_t NK = plan.NK;
   const size_t M1 = plan.M + 1;
-  // work-item size (sources), at least a bucket block's chunk
-  const unsigned item = static_cast<unsigned>(std::max(env_int("RAGNAR_SUBHIST_ITEM", static_cast<int>(kItem)), kChunk));
+  // Work-item size (sources), at least a bucket block's chunk.  The items
+  // run as one round of resident sub-histogram blocks; the full items should
+  // take about two thirds of that round's slots (the rest hold the buckets'
+  // last, partial items): measured fastest at 2^24 and at 2^26 particles
+  // (52.8 against 61.1 µs for 16384-source items at 2^24, 190 against 215 µs
+  // for 65536 at 2^26).  RAGNAR_SUBHIST_ITEM overrides it.
+  const size_t sub_slots =
+    static_cast<size_t>(num_cus()) * std::max<size_t>(1, std::min<size_t>(4, kLdsPerCU / subhist_smem(plan)));
+  const size_t auto_item = std::max<size_t>(kChunk, ceil_div(3 * n, 2 * sub_slots));
+  const unsigned item = static_cast<unsigned>(std::max<size_t>(
+    static_cast<size_t>(env_int("RAGNAR_SUBHIST_ITEM", static_cast<int>(std::min<size_t>(auto_item, 1u << 30)))),
+    kChunk));
   // items: sum over k of ceil(totals_k / item) <= n / item + NK
   const size_t max_items = n / item + NK + 1;
   float2* sortWP = static_cast<float2*>(scratch(sizeof(float2) * G * size_t{kChunk}, 6));
