@@ -83,3 +83,43 @@ def test_oracle_is_not_imported_by_the_product():
                 assert not mod.startswith("oracle") and not any(n.startswith("oracle") for n in names), f
     for f in list(root.rglob("*.cpp")) + list(root.rglob("*.hip")) + list(root.rglob("*.hpp")):
         assert "oracle" not in f.read_text().lower(), f
+
+
+def test_product_fails_loudly_without_the_hip_library(tmp_path):
+    """A copy of the package without the compiled library and module: the
+    package import, the ctypes binding and the torch wrappers all raise
+    ImportError.  There is no CPU fallback to land on."""
+    import os
+    import shutil
+    import subprocess
+    import sys
+    from pathlib import Path
+
+    src = Path(__file__).resolve().parent.parent / "ragnar_amd"
+    dst = tmp_path / "ragnar_amd"
+    dst.mkdir()
+    for f in src.glob("*.py"):
+        shutil.copy(f, dst / f.name)
+    probe = (
+        "import importlib, importlib.util, sys\n"
+        "try:\n"
+        "    import ragnar_amd\n"
+        "    print('package imported')\n"
+        "except ImportError:\n"
+        "    print('package ImportError')\n"
+        # the binding module on its own, outside the package's __init__
+        "spec = importlib.util.spec_from_file_location('capi_probe', sys.argv[1])\n"
+        "capi = importlib.util.module_from_spec(spec)\n"
+        "spec.loader.exec_module(capi)\n"
+        "try:\n"
+        "    capi.lib()\n"
+        "    print('library loaded')\n"
+        "except ImportError:\n"
+        "    print('library ImportError')\n"
+    )
+    env = {**os.environ, "PYTHONPATH": str(tmp_path)}
+    res = subprocess.run([sys.executable, "-c", probe, str(dst / "_capi.py")], cwd=tmp_path, env=env,
+                         capture_output=True, text=True, timeout=300)
+    assert res.returncode == 0, res.stderr
+    assert "package ImportError" in res.stdout, res.stdout
+    assert "library ImportError" in res.stdout, res.stdout
