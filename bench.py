@@ -836,6 +836,13 @@ def main():
     ms = event_time(step, args.steps, args.warmup, torch)
     value = evals / (ms * 1e-3)
     auto = stage_profile(T, plan, U, B, out, torch, reps=20)
+    # what a plain device copy reaches here (1 GiB read + 1 GiB written): the
+    # practical HBM ceiling the roofline fraction can be read against
+    csrc = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
+    cdst = torch.empty_like(csrc)
+    copy_ms = event_time(lambda: cdst.copy_(csrc), 5, 2, torch)
+    copy_gbps = 2 * csrc.numel() * 4 / (copy_ms * 1e-3) / 1e9
+    del csrc, cdst
     auto_method = "binned" if auto["kernels"][0][0].startswith("binned") else "direct"
 
     # the default method's output, for the checker (CPU leg)
@@ -861,12 +868,15 @@ def main():
                                "bins [1e-2, 1e7] mc^2, |u| power law p=-2.5 on [1,1e3], "
                                "B=(0,0,1)+N(0,0.3^2), unit weights",
                    "particles": n, "photon_bins": nb, "method": auto_method},
-        "roofline": dict(roofline(auto, n, nb, plan), step=step_roofline(n, ms)),
+        "roofline": dict(roofline(auto, n, nb, plan), step=step_roofline(n, ms), device_copy_GBps=copy_gbps),
         "methods_ms": method_ms,
         "methods_max_rel_diff": methods_agree,
         "stages_ms": {name: t for name, t in auto["kernels"]},
     }
 
+    rf = res["roofline"]
+    if rf.get("unit") == "GB/s":
+        rf["frac_of_device_copy"] = rf["achieved"] / copy_gbps
     progress(f"headline step {ms:.4f} ms")
     extras = {}
     if not args.no_extras:
