@@ -397,17 +397,20 @@ def test_contraction_generated_rows(orc, table, precision, ng):
 
 
 @pytest.mark.parametrize("precision", [1, 2, 3])
-@pytest.mark.parametrize("nd,ng,nb", [(1, 2, 1), (1, 64, 257), (65, 63, 1), (65, 65, 257), (3, 128, 300)])
+@pytest.mark.parametrize("nd,ng,nb", [(1, 2, 1), (1, 64, 257), (65, 63, 1), (65, 65, 257), (3, 128, 300),
+                                      (3, 2048, 300)])
 def test_contraction_small_and_ragged(orc, table, precision, nd, ng, nb):
     """The matrix-core forms at sizes around their tiles (64 rows, 64 g per
     block, 256 bins): one row, one bin, a single partial g block, a block
     and one g, 257 and 300 bins (a second, nearly empty column tile), rows
     given and generated, each row against the oracle's FP64 FromDist (1e-5 over
     bins above 1e-12 of the row's peak; F16X1, one fp16 product, 3e-3 above
-    1e-6).  The split-FP16 forms get a grid of 0.7 decades per 64 points (their
-    stated condition); split BF16, which has none, 4 decades over the grid."""
+    1e-6).  2048 g is the largest grid generated rows take (their grid terms
+    fill 48 KB of LDS).  The split-FP16 forms get at most 0.7 decades per 64
+    points (their stated condition); split BF16, which has none, 4 decades
+    over any grid."""
     rng = np.random.default_rng(nd * 1000 + ng * 10 + nb)
-    g = orc.logspace(1, 10.0 ** (4.0 if precision == 3 else 0.7 * ng / 64), ng)
+    g = orc.logspace(1, 10.0 ** (4.0 if precision == 3 else min(4.0, 0.7 * ng / 64)), ng)
     bins = orc.logspace(1e-2, 1e7, nb)
     plan = T.SyncPlan(table, bins)
     p = rng.uniform(-3.5, -1.5, nd).astype(np.float32)
