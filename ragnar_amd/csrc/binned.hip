@@ -44,6 +44,7 @@ namespace rgn {
 
 namespace {
 constexpr int kLutCells = 2048;  // sub-cell lookup cells in a sub-histogram block
+constexpr int kLog2Tab = 256;    // entries of the factorised passes' log2 table
 }  // namespace
 
 void build_binned_tables(SyncPlan& plan, const std::vector<float>& xs, const std::vector<float>& ys) {
@@ -144,6 +145,17 @@ void build_binned_tables(SyncPlan& plan, const std::vector<float>& xs, const std
   RGN_HIP(hipMemcpy(plan.binfo.ptr, bi.data(), bi.size() * sizeof(BinInfo), hipMemcpyHostToDevice));
   plan.seg.reset(seg.size() * sizeof(SegInfo));
   RGN_HIP(hipMemcpy(plan.seg.ptr, seg.data(), seg.size() * sizeof(SegInfo), hipMemcpyHostToDevice));
+  {
+    // log2 table (log2_tab): entry i holds 2 inv_i and -log2 inv_i, inv_i = RN(1 / c_i),
+    // c_i = 1 + (i + 1/2) / 256, the logarithm in long double
+    std::vector<double2> l2(kLog2Tab);
+    for (int i = 0; i < kLog2Tab; ++i) {
+      const double inv = 1.0 / (1.0 + (i + 0.5) / kLog2Tab);
+      l2[i] = make_double2(2.0 * inv, static_cast<double>(-std::log2(static_cast<long double>(inv))));
+    }
+    plan.l2tab.reset(l2.size() * sizeof(double2));
+    RGN_HIP(hipMemcpy(plan.l2tab.ptr, l2.data(), plan.l2tab.bytes, hipMemcpyHostToDevice));
+  }
 }
 
 namespace {
@@ -169,6 +181,7 @@ struct BinnedDev {
   const BinInfo* binfo;
   const SegInfo* seg;
   const float* bine;
+  const double2* l2tab;
   int M, JQ, T, kmin, NK, nbins;
   EmitConst em;
   double K;
@@ -176,17 +189,33 @@ struct BinnedDev {
 
 BinnedDev binned_view(const SyncPlan& p) {
   return BinnedDev{p.alpha.as<double>(), p.athr.as<float>(), p.jlut.as<unsigned short>(), p.sublut.as<uint2>(), p.rptr.as<int>(), p.rlist.as<int>(),
-                   p.binfo.as<BinInfo>(), p.seg.as<SegInfo>(), p.bine.as<float>(),
+                   p.binfo.as<BinInfo>(), p.seg.as<SegInfo>(), p.bine.as<float>(), p.l2tab.as<double2>(),
                    p.M, p.JQ, p.T, p.kmin, p.NK, p.nbins, emit_const(p), p.K};
 }
 
-// W and B of a particle from its OmegaSync terms; false when it contributes
-// nowhere.
+// log2 of a positive finite float to ~1e-11 absolute, from a table (in LDS):
+// x = m 2^E with m in [1, 2); the top eight fraction bits i of m pick
+// c_i = 1 + (i + 1/2) / 256 and the table's entry (2 inv_i, L_i), inv_i =
+// RN(1 / c_i), L_i = -log2 inv_i; then log2 x = E + L_i + log2(1 + r) with
+// r = m inv_i - 1 (one rounding, |r| < 2^-9) and ln(1 + r) through r^3
+// (truncation below 6e-12).  Its three FMAs take constants as multiplicands
+// or inline operands, and no division or reciprocal is needed.
+__device__ __forceinline__ double log2_tab(float xf, const double2* __restrict__ tab) {
+  int E;
+  const float mf = frexpf(xf, &E);  // [0.5, 1), m = 2 mf
+  const double2 t = tab[(__float_as_uint(mf) >> 15) & (kLog2Tab - 1)];
+  const double r = fma(static_cast<double>(mf), t.x, -1.0);
+  const double q = r * fma(r, fma(r, 1.0 / 3.0, -0.5), 1.0);  // ln(1 + r)
+  return static_cast<double>(E - 1) + fma(q, 1.4426950408889634, t.y);
+}
+
+// W and B of a particle from its OmegaSync terms (l2: the log2 table in LDS);
+// false when it contributes nowhere.
 __device__ __forceinline__ bool particle_wb(const OmegaTerms& o, const RawParticle& r, const BinnedDev& P,
-                                            float& W, double& B) {
+                                            const double2* l2, float& W, double& B) {
   W = o.chiR * r.w;
   if (!(o.e_peak > 0.0f && o.e_peak < INFINITY)) return false;  // synchrotron.hpp:162; NaN fails too
-  B = log2_pos(o.e_peak) * P.K;
+  B = log2_tab(o.e_peak, l2) * P.K;
   return B >= static_cast<double>(P.kmin) && B < static_cast<double>(P.kmin + P.NK);
 }
 
@@ -262,16 +291,20 @@ __global__ void __launch_bounds__(kThreads) fill_zero(unsigned* __restrict__ p, 
 // Pass 1: per block of kChunk particles, sources sorted by bucket k.
 //   sortWP[b * kChunk + i]: the block's sources (W, phi) in bucket order;
 //   runs[k * G + b]: run descriptor of bucket k in block b.
-template <bool HAS_E>
-__global__ void __launch_bounds__(kBucketThreads) binned_bucket(ParticlesDev D, unsigned n, BinnedDev P,
-                                                          float2* __restrict__ sortWP, unsigned* __restrict__ runs) {
+// Two blocks per CU without E fields (64 VGPRs), one with them.  NT: the
+// particles are read with non-temporal loads (see sync_forward_binned).
+template <bool HAS_E, bool NT>
+__global__ void __launch_bounds__(kBucketThreads) __attribute__((amdgpu_waves_per_eu(HAS_E ? 4 : 8)))
+binned_bucket(ParticlesDev D, unsigned n, BinnedDev P, float2* __restrict__ sortWP, unsigned* __restrict__ runs) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int NK = P.NK;
-  unsigned* hist = reinterpret_cast<unsigned*>(smem);
+  double2* l2 = reinterpret_cast<double2*>(smem);
+  unsigned* hist = reinterpret_cast<unsigned*>(l2 + kLog2Tab);
   unsigned* off = hist + NK;
   unsigned* part = off + NK;
   float2* sWP = reinterpret_cast<float2*>(part + kBucketThreads / 64);  // after an even word count: 8-byte aligned
   for (int i = threadIdx.x; i < NK; i += kBucketThreads) hist[i] = 0u;
+  if (threadIdx.x < kLog2Tab) l2[threadIdx.x] = P.l2tab[threadIdx.x];
   __syncthreads();
   const unsigned G = gridDim.x;
   const unsigned base = blockIdx.x * static_cast<unsigned>(kChunk);
@@ -280,11 +313,11 @@ __global__ void __launch_bounds__(kBucketThreads) binned_bucket(ParticlesDev D, 
   unsigned rk[kPer];
   unsigned p = base + threadIdx.x;
   RawParticle raw{};
-  if (p < n) raw = load_particle32(D, p);
+  if (p < n) raw = load_particle32<NT>(D, p);
 #pragma unroll
   for (int i = 0; i < kPer; ++i, p += kBucketThreads) {
     const RawParticle cur = raw;
-    if (i + 1 < kPer && p + kBucketThreads < n) raw = load_particle32(D, p + kBucketThreads);
+    if (i + 1 < kPer && p + kBucketThreads < n) raw = load_particle32<NT>(D, p + kBucketThreads);
     kk[i] = -1;
     Wr[i] = 0.0f;
     ph[i] = 0.0f;
@@ -295,7 +328,7 @@ __global__ void __launch_bounds__(kBucketThreads) binned_bucket(ParticlesDev D, 
     // e_peak round to the reference's floats except within ~2 double ulps of
     // a float rounding boundary, where one float ulp moves W and B; the
     // binned sum never depended on them bit for bit
-    if (p < n && particle_wb(omega_terms_approx<HAS_E>(cur, P.em), cur, P, W, B)) {
+    if (p < n && particle_wb(omega_terms_approx<HAS_E>(cur, P.em), cur, P, l2, W, B)) {
       const double fl = floor(B);
       kk[i] = static_cast<int>(fl) - P.kmin;
       Wr[i] = W;
@@ -771,9 +804,11 @@ __global__ void __launch_bounds__(STAGE ? kGather : kThreads) binned_adj_gather(
   constexpr int NT_ = STAGE ? kGather : kThreads;
   extern __shared__ __align__(16) unsigned char smem[];
   double2* urows = reinterpret_cast<double2*>(smem);  // STAGE: kstage rows of M + 1 (or the histogram)
-  double* alpha = reinterpret_cast<double*>(smem + (STAGE ? kStageBytes : 0));
+  double2* l2 = reinterpret_cast<double2*>(smem + (STAGE ? kStageBytes : 0));
+  double* alpha = reinterpret_cast<double*>(l2 + kLog2Tab);
   unsigned short* jlut = reinterpret_cast<unsigned short*>(alpha + P.M);
   __shared__ int s_w0;
+  for (int i = threadIdx.x; i < kLog2Tab; i += NT_) l2[i] = P.l2tab[i];
   for (int i = threadIdx.x; i < P.M; i += NT_) alpha[i] = P.alpha[i];
   for (int i = threadIdx.x; i < P.JQ; i += NT_) jlut[i] = P.jlut[i];
   const double dlnx = P.K / 0.69314718055994530942;  // dt / dln x
@@ -791,7 +826,7 @@ __global__ void __launch_bounds__(STAGE ? kGather : kThreads) binned_adj_gather(
       const RawParticle r = load_particle32<false>(D, p);
       float W;
       double B;
-      if (particle_wb(omega_terms_approx<HAS_E>(r, P.em), r, P, W, B)) {
+      if (particle_wb(omega_terms_approx<HAS_E>(r, P.em), r, P, l2, W, B)) {
         atomicAdd(hist + (static_cast<int>(floor(B)) - P.kmin), 1u);
       }
     }
@@ -835,16 +870,20 @@ __global__ void __launch_bounds__(STAGE ? kGather : kThreads) binned_adj_gather(
   unsigned p = blockIdx.x * NT_ + threadIdx.x;
   // NT: particles and gradients stream through once (non-temporal), so that
   // the table lookups keep the caches
-  RawParticle raw{};
-  if (p < nn) raw = load_particle32<NT>(D, p);
+  // two particles ahead in flight per thread (the gather is limited by memory
+  // latency at two blocks per CU)
+  RawParticle r1{}, r2{};
+  if (p < nn) r1 = load_particle32<NT>(D, p);
+  if (p + stride < nn) r2 = load_particle32<NT>(D, p + stride);
   for (; p < nn; p += stride) {
-    const RawParticle cur = raw;
-    if (p + stride < nn) raw = load_particle32<NT>(D, p + stride);
+    const RawParticle cur = r1;
+    r1 = r2;
+    if (p + 2 * stride < nn) r2 = load_particle32<NT>(D, p + 2 * stride);
     float W;
     double B;
     ParticleGrads g{};
     const OmegaTerms o = omega_terms_approx<HAS_E>(cur, P.em);
-    if (particle_wb(o, cur, P, W, B)) {
+    if (particle_wb(o, cur, P, l2, W, B)) {
       const double fl = floor(B);
       const double phi = B - fl;
       const int kk = static_cast<int>(fl) - P.kmin;
@@ -872,10 +911,11 @@ __global__ void __launch_bounds__(STAGE ? kGather : kThreads) binned_adj_gather(
 
 }  // namespace
 
-// LDS bytes of the bucket pass for a plan (hist, offsets, scan partials and
-// the block's sorted sources).
+// LDS bytes of the bucket pass for a plan (log2 table, hist, offsets, scan
+// partials and the block's sorted sources).
 static size_t bucket_smem(const SyncPlan& plan) {
-  return sizeof(unsigned) * (2 * static_cast<size_t>(plan.NK) + kBucketThreads / 64) + 2 * sizeof(float) * kChunk;
+  return sizeof(double2) * kLog2Tab + sizeof(unsigned) * (2 * static_cast<size_t>(plan.NK) + kBucketThreads / 64) +
+         2 * sizeof(float) * kChunk;
 }
 
 static size_t subhist_smem(const SyncPlan& plan) {
@@ -944,10 +984,19 @@ void sync_forward_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, 
   const size_t clean_after = std::max(runs_clean, NK * G);
   runs_clean = 0;
   const bool has_e = D.e[0] || D.e[1] || D.e[2];
+  // Particles larger than the device's last-level cache (256 MiB on MI355X)
+  // are read non-temporally: they would not be found there on the next call
+  // anyway, and the bucket pass ran 107 -> 100 us at 2^24 particles (403 MB).
+  const size_t in_bytes = sizeof(float) * n * (6 + (has_e ? 3 : 0) + (D.w ? 1 : 0));
+  const bool nt = in_bytes > (size_t{256} << 20);
+  const unsigned nn = static_cast<unsigned>(n);
+  const size_t bsm = bucket_smem(plan);
   if (has_e) {
-    binned_bucket<true><<<G, kBucketThreads, bucket_smem(plan), st>>>(D, static_cast<unsigned>(n), P, sortWP, runs);
+    if (nt) binned_bucket<true, true><<<G, kBucketThreads, bsm, st>>>(D, nn, P, sortWP, runs);
+    else binned_bucket<true, false><<<G, kBucketThreads, bsm, st>>>(D, nn, P, sortWP, runs);
   } else {
-    binned_bucket<false><<<G, kBucketThreads, bucket_smem(plan), st>>>(D, static_cast<unsigned>(n), P, sortWP, runs);
+    if (nt) binned_bucket<false, true><<<G, kBucketThreads, bsm, st>>>(D, nn, P, sortWP, runs);
+    else binned_bucket<false, false><<<G, kBucketThreads, bsm, st>>>(D, nn, P, sortWP, runs);
   }
   RGN_HIP(hipGetLastError());
   stage_mark(st, "binned_bucket");
@@ -994,7 +1043,8 @@ void sync_adjoint_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, 
   }
   BinnedDev Pg = P;
   if (plan.NK == 0) Pg.NK = 0;  // every particle falls outside: gradients are zero
-  const size_t tables = sizeof(double) * std::max(plan.M, 1) + sizeof(unsigned short) * plan.JQ;
+  const size_t tables = sizeof(double2) * kLog2Tab + sizeof(double) * std::max(plan.M, 1) +
+                        sizeof(unsigned short) * plan.JQ;
   // The general form also for stores without E: the shorter one measured ~4%
   // slower here (more resident waves, same memory traffic).  Non-temporal
   // streams pay off once a call's particles and gradients outgrow the

@@ -46,6 +46,7 @@ struct SyncPlan {
   DeviceBuffer rlist;              //   rlist[rptr[r] .. rptr[r + 1]) (ascending e)
   DeviceBuffer binfo;              // BinInfo[nbins]
   DeviceBuffer seg;                // SegInfo[T - 1]
+  DeviceBuffer l2tab;              // double2[256]: the factorised passes' log2 table
 };
 
 // The plan's emission constants with the correctly rounded reciprocals of the
