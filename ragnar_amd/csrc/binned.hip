@@ -493,6 +493,8 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
   unsigned* wro = reinterpret_cast<unsigned*>(athr + P.M);   // kWin + 1
   unsigned* wlo = wro + kWin + 1;                              // kWin
   __shared__ unsigned s_item[2], s_wsum[kWaves];
+  // per wave: run starts marked by position within a 64-position step (see step)
+  __shared__ unsigned s_mark[kWaves][64];
   const unsigned it = blockIdx.x;
   int k;
   unsigned first;
@@ -503,6 +505,7 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
   }
   for (int i = threadIdx.x; i < P.M; i += kSubThreads) athr[i] = P.athr[i];
   for (int c = threadIdx.x; c < JQ2; c += kSubThreads) lut[c] = P.sublut[c];
+  s_mark[threadIdx.x >> 6][threadIdx.x & 63] = 0u;
   __syncthreads();
   const unsigned v0 = (it - first) * item;
   const unsigned v1 = min(totals[k], v0 + item);
@@ -542,9 +545,11 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
       const unsigned pa = min(vb, va + per * wave), pb = min(vb, pa + per);
       // The wave's positions go 64 at a time, lane l taking v00 + l.  c0 is
       // the run holding v00 (wave-uniform).  Each lane reads one run start
-      // past c0; those inside the next 64 positions become bits of a
-      // position bitmap, and a lane's run is c0 plus the starts at or below
-      // its position: no per-lane search.
+      // past c0; those inside the next 64 positions mark their offset in the
+      // wave's LDS row (with a tag unique to the step, so the row is never
+      // cleared), a ballot of the row gives the position bitmap, and a
+      // lane's run is c0 plus the starts at or below its position: no
+      // per-lane search and no loop over the starts.
       int c0 = 0;
       if (pa < pb) {
         int lo = 0, hi = nc - 1;  // last run starting at or before pa
@@ -554,16 +559,16 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
         }
         c0 = lo;
       }
+      unsigned* mark = s_mark[wave];
       auto step = [&](unsigned v00) {  // every lane calls it; returns the lane's run
         const int c = c0 + 1 + lane;
-        const unsigned sc = c <= nc ? crs[c] : 0xffffffffu;
-        const unsigned long long in = __ballot(sc <= v00 + 64);
-        const int L = __popcll(in);  // run starts in (v00, v00 + 64]: held by lanes 0 .. L - 1
-        unsigned long long bits = 0;
-        for (int i = 0; i < L; ++i) {
-          const unsigned si = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(sc), i));
-          if (si - v00 < 64u) bits |= 1ull << (si - v00);
-        }
+        const unsigned sc = c <= nc ? crs[c] : 0xffffffffu;  // > v00
+        const int L = __popcll(__ballot(sc <= v00 + 64));  // starts in (v00, v00 + 64]
+        const unsigned tag = v00 + 1;  // positions only grow: unique per step, never 0
+        if (sc - v00 - 1 < 63u) mark[sc - v00] = tag;       // starts at v00 + 1 .. v00 + 63
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const unsigned long long bits = __ballot(mark[lane] == tag);
         const int run = c0 + __popcll(bits & ((2ull << lane) - 1));
         c0 += L;
         return min(run, nc - 1);  // past the window end only for positions >= pb
@@ -945,8 +950,8 @@ void sync_forward_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, 
   // last, partial items): measured fastest at 2^24 and at 2^26 particles
   // (52.8 against 61.1 µs for 16384-source items at 2^24, 190 against 215 µs
   // for 65536 at 2^26).  RAGNAR_SUBHIST_ITEM overrides it.
-  const size_t sub_slots =
-    static_cast<size_t>(num_cus()) * std::max<size_t>(1, std::min<size_t>(4, kLdsPerCU / subhist_smem(plan)));
+  const size_t sub_lds = subhist_smem(plan) + sizeof(unsigned) * (kSubThreads + 16);  // + static LDS
+  const size_t sub_slots = static_cast<size_t>(num_cus()) * std::max<size_t>(1, std::min<size_t>(4, kLdsPerCU / sub_lds));
   const size_t auto_item = std::max<size_t>(kChunk, ceil_div(3 * n, 2 * sub_slots));
   const unsigned item = static_cast<unsigned>(std::max<size_t>(
     static_cast<size_t>(env_int("RAGNAR_SUBHIST_ITEM", static_cast<int>(std::min<size_t>(auto_item, 1u << 30)))),
