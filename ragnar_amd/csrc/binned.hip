@@ -490,11 +490,11 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
   const int JQ2 = min(P.JQ, kLutCells);
   uint2* lut = reinterpret_cast<uint2*>(m1 + M1);            // JQ2
   float* athr = reinterpret_cast<float*>(lut + JQ2);
-  unsigned* wro = reinterpret_cast<unsigned*>(athr + P.M);   // kWin + 1
-  unsigned* wlo = wro + kWin + 1;                              // kWin
+  unsigned* wro = reinterpret_cast<unsigned*>(athr + P.M);   // kWin + 65
+  unsigned* wlo = wro + kWin + 65;                             // kWin
   __shared__ unsigned s_item[2], s_wsum[kWaves];
   // per wave: run starts marked by position within a 64-position step (see step)
-  __shared__ unsigned s_mark[kWaves][64];
+  __shared__ unsigned s_mark[kWaves][65];
   const unsigned it = blockIdx.x;
   int k;
   unsigned first;
@@ -505,7 +505,7 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
   }
   for (int i = threadIdx.x; i < P.M; i += kSubThreads) athr[i] = P.athr[i];
   for (int c = threadIdx.x; c < JQ2; c += kSubThreads) lut[c] = P.sublut[c];
-  s_mark[threadIdx.x >> 6][threadIdx.x & 63] = 0u;
+  s_mark[threadIdx.x >> 6][threadIdx.x & 63] = 0u;  // entry 64 is only written
   __syncthreads();
   const unsigned v0 = (it - first) * item;
   const unsigned v1 = min(totals[k], v0 + item);
@@ -527,16 +527,18 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
   for (int ws = rlo; ws <= rhi; ws += kWin) {
     const int nc = min(rhi - ws + 1, kWin);
     // the window's runs clipped to [v0, v1): run c covers virtual
-    // [crs[c], crs[c + 1]) from physical index cix[c]
-    unsigned* crs = wro;   // nc + 1 entries
-    unsigned* cix = wlo;   // nc entries
+    // [crs[c], crs[c + 1]), and position v of it is physical index v + dlt[c];
+    // 64 entries of ~0 follow crs[nc], so a step reads starts without a check
+    unsigned* crs = wro;   // nc + 65 entries
+    unsigned* dlt = wlo;   // nc entries
     for (int c = threadIdx.x; c < nc; c += kSubThreads) {
       const unsigned r0 = vo[ws + c];
       const unsigned ra = max(r0, v0);
       crs[c] = ra;
-      cix[c] = ci[ws + c] + (ra - r0);
+      dlt[c] = ci[ws + c] - r0;  // modulo 2^32
     }
     if (threadIdx.x == 0) crs[nc] = min(vo[ws + nc], v1);
+    if (threadIdx.x < 64) crs[nc + 1 + threadIdx.x] = 0xffffffffu;
     __syncthreads();
     if (nc > 0) {
       // wave w walks positions [pa, pb) of the window, lane-interleaved
@@ -561,11 +563,13 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
       }
       unsigned* mark = s_mark[wave];
       auto step = [&](unsigned v00) {  // every lane calls it; returns the lane's run
-        const int c = c0 + 1 + lane;
-        const unsigned sc = c <= nc ? crs[c] : 0xffffffffu;  // > v00
+        const unsigned sc = crs[c0 + 1 + lane];  // > v00 (c0 <= nc)
         const int L = __popcll(__ballot(sc <= v00 + 64));  // starts in (v00, v00 + 64]
         const unsigned tag = v00 + 1;  // positions only grow: unique per step, never 0
-        if (sc - v00 - 1 < 63u) mark[sc - v00] = tag;       // starts at v00 + 1 .. v00 + 63
+        // starts at v00 + 1 .. v00 + 63 mark their offset; the others write
+        // the unread entry 64
+        const unsigned d = sc - v00;
+        mark[d - 1 < 63u ? d : 64u] = tag;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         const unsigned long long bits = __ballot(mark[lane] == tag);
@@ -590,7 +594,7 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
         for (int q = 0; q < kBatch; ++q) {
           const unsigned v = v00 + q * 64 + lane;
           const int r = step(v00 + q * 64);
-          idx[q] = cix[r] + (v - crs[r]);
+          idx[q] = v + dlt[r];
         }
         float2 wp[kBatch];
 #pragma unroll
@@ -604,7 +608,7 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
         for (int q = 0; q < kBatch; ++q) {
           const unsigned v = v00 + q * 64 + lane;
           const int r = step(v00 + q * 64);
-          idx[q] = v < pb ? cix[r] + (v - crs[r]) : ~0u;
+          idx[q] = v < pb ? v + dlt[r] : ~0u;
         }
         float2 wp[kBatch];
 #pragma unroll
@@ -926,7 +930,7 @@ static size_t bucket_smem(const SyncPlan& plan) {
 static size_t subhist_smem(const SyncPlan& plan) {
   const size_t M1 = plan.M + 1;
   return sizeof(double) * 2 * M1 + sizeof(uint2) * std::min(plan.JQ, kLutCells) + sizeof(float) * plan.M +
-         sizeof(unsigned) * (2 * kWin + 1);
+         sizeof(unsigned) * (2 * kWin + 65);
 }
 
 void sync_forward_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, float* out, hipStream_t st) {
@@ -950,7 +954,8 @@ void sync_forward_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, 
   // last, partial items): measured fastest at 2^24 and at 2^26 particles
   // (52.8 against 61.1 µs for 16384-source items at 2^24, 190 against 215 µs
   // for 65536 at 2^26).  RAGNAR_SUBHIST_ITEM overrides it.
-  const size_t sub_lds = subhist_smem(plan) + sizeof(unsigned) * (kSubThreads + 16);  // + static LDS
+  // + the static LDS (marks, item and scan words, with alignment)
+  const size_t sub_lds = subhist_smem(plan) + sizeof(unsigned) * (kSubThreads / 64 * 66 + 2) + 64;
   const size_t sub_slots = static_cast<size_t>(num_cus()) * std::max<size_t>(1, std::min<size_t>(4, kLdsPerCU / sub_lds));
   const size_t auto_item = std::max<size_t>(kChunk, ceil_div(3 * n, 2 * sub_slots));
   const unsigned item = static_cast<unsigned>(std::max<size_t>(
