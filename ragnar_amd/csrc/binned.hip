@@ -562,23 +562,29 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
         c0 = lo;
       }
       unsigned* mark = s_mark[wave];
-      auto step = [&](unsigned v00) {  // every lane calls it; returns the lane's run
-        const unsigned sc = crs[c0 + 1 + lane];  // > v00 (c0 <= nc)
+      const unsigned long long below = (2ull << lane) - 1;  // lanes 0 .. lane (all for lane 63)
+      unsigned sc = crs[c0 + 1 + lane];  // the lane's start past c0 (> every position of run c0)
+      // every lane calls it; returns the lane's run (past the window end, nc,
+      // only for positions >= pb)
+      auto step = [&](unsigned v00) {
         const int L = __popcll(__ballot(sc <= v00 + 64));  // starts in (v00, v00 + 64]
-        const unsigned tag = v00 + 1;  // positions only grow: unique per step, never 0
-        // starts at v00 + 1 .. v00 + 63 mark their offset; the others write
-        // the unread entry 64
-        const unsigned d = sc - v00;
-        mark[d - 1 < 63u ? d : 64u] = tag;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const unsigned long long bits = __ballot(mark[lane] == tag);
-        const int run = c0 + __popcll(bits & ((2ull << lane) - 1));
-        c0 += L;
-        return min(run, nc - 1);  // past the window end only for positions >= pb
+        int run = c0;
+        if (L) {  // wave-uniform; none in most steps of the long runs
+          const unsigned tag = v00 + 1;  // positions only grow: unique per step, never 0
+          // starts at v00 + 1 .. v00 + 63 mark their offset; the others write
+          // the unread entry 64
+          const unsigned d = sc - v00;
+          mark[d - 1 < 63u ? d : 64u] = tag;
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+          run += __popcll(__ballot(mark[lane] == tag) & below);
+          c0 += L;
+          sc = crs[c0 + 1 + lane];  // c0 <= nc
+        }
+        return run;
       };
-      auto accumulate = [&](float2 wp) {
-        const uint2 e = lut[min(static_cast<int>(wp.y * JQ2), JQ2 - 1)];
+      auto accumulate = [&](float2 wp) {  // 0 <= phi < 1: the cell index is below JQ2
+        const uint2 e = lut[static_cast<int>(wp.y * JQ2)];
         int j = static_cast<int>(e.x & 0xffffu) + (__uint_as_float(e.y) <= wp.y);
         if (e.x >> 16) {  // rare: a cell with several alphas
           while (j < P.M && athr[j] <= wp.y) ++j;
@@ -607,7 +613,7 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
 #pragma unroll
         for (int q = 0; q < kBatch; ++q) {
           const unsigned v = v00 + q * 64 + lane;
-          const int r = step(v00 + q * 64);
+          const int r = min(step(v00 + q * 64), nc - 1);
           idx[q] = v < pb ? v + dlt[r] : ~0u;
         }
         float2 wp[kBatch];
