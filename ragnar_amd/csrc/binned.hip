@@ -168,7 +168,7 @@ static_assert(kChunk <= 32768, "run descriptors pack start and length in 16 bits
 constexpr size_t kLdsPerCU = 160 * 1024;   // LDS of a gfx950 CU
 constexpr int kSubThreads = 512;           // threads of a sub-histogram work item
 constexpr int kWin = 2048;                 // runs per LDS window of a work item
-constexpr int kPrefThreads = 768;          // 3 item groups x 256 sub-cells per prefix block (two blocks per CU)
+constexpr int kPrefThreads = 768;          // threads of a prefix block (two blocks per CU)
 constexpr int kRowThreads = 1024;          // one row of run descriptors per scan block
 
 struct BinnedDev {
@@ -634,51 +634,56 @@ __global__ void __launch_bounds__(kSubThreads) binned_subhist(BinnedDev P, int G
   }
 }
 
-// Pass 4: per bucket, its items summed (three interleaved item groups, combined
-// in a fixed order), then inclusive prefix sums over sub-cells:
-// P0[k][j] = sum_{j' <= j} m0, P1 likewise.
+// Pass 4: per bucket, its items summed (interleaved item groups, combined in a
+// fixed order), then inclusive prefix sums over sub-cells:
+// P0[k][j] = sum_{j' <= j} m0, P1 likewise.  The block's threads form as many
+// groups of C = M + 1 rounded up to 64 columns as fit (two of 320 for 256
+// bins), so a pass covers every sub-cell and each group sums a share of the
+// items.
 __global__ void __launch_bounds__(kPrefThreads) binned_prefix(int M, unsigned item, const unsigned* __restrict__ totals,
                                                               const double* __restrict__ partial,
                                                               double* __restrict__ P0, double* __restrict__ P1) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double part[kPrefThreads / 64];
-  constexpr int kGroups = kPrefThreads / 256;
-  __shared__ double red[2][kGroups][256];
+  __shared__ double red[2][kPrefThreads];  // [m0, m1][group * C + column]
   const int M1 = M + 1;
   double* s0 = reinterpret_cast<double*>(smem);
   double* s1 = s0 + M1;
   const int k = blockIdx.x;
   __shared__ unsigned s_wsum[kPrefThreads / 64];
   const unsigned i0 = items_before<kPrefThreads>(totals, k, item, s_wsum), i1 = i0 + items_of(totals[k], item);
-  const int grp = threadIdx.x >> 8, col = threadIdx.x & 255;
-  for (int j0 = 0; j0 < M1; j0 += 256) {
+  const int C = min((M1 + 63) & ~63, kPrefThreads);
+  const int groups = kPrefThreads / C;
+  const int grp = static_cast<int>(threadIdx.x) / C, col = static_cast<int>(threadIdx.x) - grp * C;
+  for (int j0 = 0; j0 < M1; j0 += C) {
     const int j = j0 + col;
-    // this group's items it = i0 + grp + 4 q, four of them per pass with their
-    // loads in flight together, summed in a fixed order
+    // this group's items it = i0 + grp + groups q, four of them per pass with
+    // their loads in flight together, summed in a fixed order
     double a[4] = {0.0, 0.0, 0.0, 0.0}, b[4] = {0.0, 0.0, 0.0, 0.0};
-    if (j < M1) {
+    if (grp < groups && j < M1) {
       unsigned it = i0 + grp;
-      for (; it + 3 * kGroups < i1; it += 4 * kGroups) {
+      for (; it + 3 * groups < i1; it += 4 * groups) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          a[q] += partial[static_cast<size_t>(it + kGroups * q) * 2 * M1 + j];
-          b[q] += partial[static_cast<size_t>(it + kGroups * q) * 2 * M1 + M1 + j];
+          a[q] += partial[static_cast<size_t>(it + groups * q) * 2 * M1 + j];
+          b[q] += partial[static_cast<size_t>(it + groups * q) * 2 * M1 + M1 + j];
         }
       }
-      for (int q = 0; it < i1; it += kGroups, ++q) {  // at most three left
+      for (int q = 0; it < i1; it += groups, ++q) {  // at most three left
         a[q] += partial[static_cast<size_t>(it) * 2 * M1 + j];
         b[q] += partial[static_cast<size_t>(it) * 2 * M1 + M1 + j];
       }
     }
-    red[0][grp][col] = (a[0] + a[1]) + (a[2] + a[3]);
-    red[1][grp][col] = (b[0] + b[1]) + (b[2] + b[3]);
+    if (grp < groups) {
+      red[0][threadIdx.x] = (a[0] + a[1]) + (a[2] + a[3]);
+      red[1][threadIdx.x] = (b[0] + b[1]) + (b[2] + b[3]);
+    }
     __syncthreads();
     if (grp == 0 && j < M1) {
-      double t0 = red[0][0][col], t1 = red[1][0][col];
-#pragma unroll
-      for (int g = 1; g < kGroups; ++g) {
-        t0 += red[0][g][col];
-        t1 += red[1][g][col];
+      double t0 = red[0][col], t1 = red[1][col];
+      for (int g = 1; g < groups; ++g) {
+        t0 += red[0][g * C + col];
+        t1 += red[1][g * C + col];
       }
       s0[j] = t0;
       s1[j] = t1;
