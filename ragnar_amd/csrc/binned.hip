@@ -813,9 +813,11 @@ struct GradPtrs {
 // kStage-row window holding most of them into LDS (a power-law population
 // puts most particles in a few consecutive buckets); particles in the window
 // read their entry from LDS, the rest from the table in memory (L2).  The
-// random 16-byte reads from L2 otherwise move a whole cache line each.
-constexpr int kGather = 512;                 // threads of a staged gather block
-constexpr size_t kStageBytes = 60 * 1024;    // LDS for the staged U rows
+// random 16-byte reads from L2 otherwise move a whole cache line each.  One
+// 1024-thread block per CU with 120 KB of rows (29 at 256 bins) measured
+// 197 us at 2^24 particles, two 512-thread blocks with 60 KB each 205 us.
+constexpr int kGather = 1024;                // threads of a staged gather block
+constexpr size_t kStageBytes = 120 * 1024;   // LDS for the staged U rows
 
 template <bool HAS_E, bool NT, bool STAGE>
 __global__ void __launch_bounds__(STAGE ? kGather : kThreads) binned_adj_gather(ParticlesDev D, size_t n, BinnedDev P,
@@ -1080,7 +1082,7 @@ void sync_adjoint_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, 
                      env_int("RAGNAR_GATHER_STAGE", 1) == 1;
   if (stage) {
     const int grid = static_cast<int>(std::max<size_t>(
-      1, std::min<size_t>(ceil_div(n, kGather), static_cast<size_t>(num_cus()) * env_int("RAGNAR_GATHER_SBPC", 2))));
+      1, std::min<size_t>(ceil_div(n, kGather), static_cast<size_t>(num_cus()) * env_int("RAGNAR_GATHER_SBPC", 1))));
     const size_t smem = kStageBytes + tables;
     if (nt) {
       binned_adj_gather<true, true, true><<<grid, kGather, smem, st>>>(D, n, Pg, U, G, kstage);
