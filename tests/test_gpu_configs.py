@@ -142,6 +142,13 @@ def test_fit_2p26_gradient_subset(orc, table):
     Wc = dev(w_cur)
     target = T.sync_spectrum(plan, U, B, dev(workloads.fit_weights(u, -2.2, 3e3))).double()
     S = T.sync_spectrum(plan, U, B, Wc).double()
+    # the factorised forward at this size (items spanning several LDS windows
+    # of runs) against the pair-by-pair kernel
+    Sd = T.sync_spectrum(plan, U, B, Wc, method="direct").double()
+    fl = Sd > 1e-12 * Sd.max()
+    ferr = torch.max(torch.abs(S - Sd)[fl] / Sd[fl]).item()
+    print(f"forward at 2^26, default vs pair by pair: {ferr:.2e}")
+    assert ferr < 5e-6
     live = (S > 0) & (target > 0)
     r = torch.where(live, S.clamp_min(1e-300).log() - target.clamp_min(1e-300).log(), torch.zeros_like(S))
     gbar = torch.where(live, 2.0 * r / S.clamp_min(1e-300), torch.zeros_like(S)).float()
