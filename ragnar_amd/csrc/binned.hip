@@ -713,13 +713,14 @@ __device__ __forceinline__ void seg_line(const BinnedDev& P, const BinInfo& b, i
   C = g.y + (b.A - k_abs - g.tau) * g.sigma;
 }
 
-// Pass 5: S_e = e_e sum_k (...), one block per bin, fixed-order reduction.
+// Pass 5: S_e = e_e sum_k (...), one block per bin (a thread per bucket),
+// fixed-order reduction.
 __global__ void __launch_bounds__(kThreads) binned_evaluate(BinnedDev P, const double* __restrict__ P0,
                                                             const double* __restrict__ P1,
                                                             float* __restrict__ out) {
   const int e = blockIdx.x;
   const BinInfo b = P.binfo[e];
-  __shared__ double red[kThreads];
+  __shared__ double red[kThreads / 64];
   const int M1 = P.M + 1;
   double acc = 0.0;
   if (b.r >= 0) {
@@ -736,15 +737,16 @@ __global__ void __launch_bounds__(kThreads) binned_evaluate(BinnedDev P, const d
       acc += (Cp * p0 - sp * p1) + (Cm * (t0 - p0) - sm * (t1 - p1));
     }
   }
-  red[threadIdx.x] = acc;
+  // a fixed shuffle tree per wave, then the four wave sums in order
+#pragma unroll
+  for (int w = 32; w > 0; w >>= 1) acc += __shfl_xor(acc, w);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
   __syncthreads();
-  for (int w = kThreads / 2; w > 0; w >>= 1) {
-    if (static_cast<int>(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
-  }
   if (threadIdx.x == 0) {
+    double t = red[0];
+    for (int w = 1; w < kThreads / 64; ++w) t += red[w];
     const float ee = P.bine[e];
-    out[e] = (b.r < 0 || ee == 0.0f) ? 0.0f : static_cast<float>(static_cast<double>(ee) * red[0]);
+    out[e] = (b.r < 0 || ee == 0.0f) ? 0.0f : static_cast<float>(static_cast<double>(ee) * t);
   }
 }
 
