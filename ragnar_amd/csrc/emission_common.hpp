@@ -184,8 +184,12 @@ __device__ __forceinline__ double rcp_nr(double x) {
   return fma(y, fma(-x, y, 1.0), y);
 }
 
-// sqrt(x) to about one ulp for finite x >= 0 (hardware reciprocal square root
-// estimate and two Goldschmidt steps); x = 0 gives 0, negative x gives NaN.
+// sqrt(x) to about one ulp for finite x > 0 (hardware reciprocal square root
+// estimate and two Goldschmidt steps), without special cases: x = 0 and
+// x = inf give NaN (so do negative and NaN x).  omega_terms_approx uses it:
+// there gamma^2 >= 1, and a zero rootQ (chiR = 0, e_peak = 0) and a NaN one
+// both leave e_peak outside (0, inf), which drops the particle, so the NaN is
+// never used.
 __device__ __forceinline__ double sqrt_nr(double x) {
   const double y = __builtin_amdgcn_rsq(x);
   double g = x * y, h = 0.5 * y;
@@ -193,8 +197,7 @@ __device__ __forceinline__ double sqrt_nr(double x) {
   g = fma(g, r, g);
   h = fma(h, r, h);
   r = fma(-g, h, 0.5);
-  g = fma(g, r, g);
-  return x > 0.0 ? g : (x == 0.0 ? 0.0 : __builtin_nan(""));
+  return fma(g, r, g);
 }
 
 // The OmegaSync terms to about one ulp per operation, for the adjoint's
