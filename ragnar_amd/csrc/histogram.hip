@@ -264,10 +264,12 @@ HistShape hist_shape(const rgn_hist_plan& plan, const float* const d_u[3], size_
   sh.vec = aligned16(d_u[0]) && aligned16(d_u[1]) && aligned16(d_u[2]);
   const size_t per_thread = sh.vec ? 8 : 1;
   const size_t tiles = ceil_div(std::max<size_t>(n, 1), kThreads * per_thread);
-  // lane groups per wave: as many private copies as fit in 48 KB of LDS
+  // lane groups per wave: as many private copies as fit in 24 KB of LDS (at
+  // 200 bins two per wave, and seven blocks share a CU: 51 -> 50 us at 2^24
+  // against four per wave in 48 KB)
   const size_t per_copy = static_cast<size_t>(nb + 1) * (sizeof(double) + sizeof(unsigned));
   sh.qper = 4;
-  while (sh.qper > 1 && 4 * sh.qper * per_copy + (nb + 1) * sizeof(double) > 48 * 1024) sh.qper /= 2;
+  while (sh.qper > 1 && 4 * sh.qper * per_copy + (nb + 1) * sizeof(double) > 24 * 1024) sh.qper /= 2;
   sh.smem = 4 * sh.qper * per_copy + (nb + 1) * sizeof(double);
   // one round of blocks: as many per CU as the LDS holds (at most 8), so the
   // per-block clearing and reduction of the private copies is paid once
