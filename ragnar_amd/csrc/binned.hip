@@ -818,6 +818,8 @@ struct GradPtrs {
 // random 16-byte reads from L2 otherwise move a whole cache line each.  One
 // 1024-thread block per CU with 120 KB of rows (29 at 256 bins) measured
 // 197 us at 2^24 particles, two 512-thread blocks with 60 KB each 205 us.
+// HAS_E = false: the store has no E and dL/dE is not wanted (the E terms
+// drop out of OmegaSync and of the gradients).
 constexpr int kGather = 1024;                // threads of a staged gather block
 constexpr size_t kStageBytes = 120 * 1024;   // LDS for the staged U rows
 
@@ -915,7 +917,7 @@ __global__ void __launch_bounds__(STAGE ? kGather : kThreads) binned_adj_gather(
       const double2 u = (STAGE && kk >= w0 && kk < w1) ? urows[static_cast<size_t>(kk - w0) * M1 + j]
                                                        : U[static_cast<size_t>(kk) * M1 + j];
       // sum_e g'_e F and sum_e g'_e dF/dln x at this particle
-      g = emission_grads(o, cur, u.x - u.y * phi, dlnx * u.y, P.em);
+      g = emission_grads<HAS_E>(o, cur, u.x - u.y * phi, dlnx * u.y, P.em);
     }
     const double gw = g.gw, gux = g.gu[0], guy = g.gu[1], guz = g.gu[2];
     const double gbx = g.gb[0], gby = g.gb[1], gbz = g.gb[2];
@@ -1070,8 +1072,11 @@ void sync_adjoint_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, 
   if (plan.NK == 0) Pg.NK = 0;  // every particle falls outside: gradients are zero
   const size_t tables = sizeof(double2) * kLog2Tab + sizeof(double) * std::max(plan.M, 1) +
                         sizeof(unsigned short) * plan.JQ;
-  // The general form also for stores without E: the shorter one measured ~4%
-  // slower here (more resident waves, same memory traffic).  Non-temporal
+  // The unstaged form is the general one also for stores without E: the
+  // shorter one measured ~4% slower there (more resident waves, same memory
+  // traffic).  The staged form drops the E terms when the store has no E and
+  // dL/dE is not wanted (35 of 253 vector instructions per particle,
+  // 195 -> 188 us at 2^24 particles).  Non-temporal
   // streams pay off once a call's particles and gradients outgrow the
   // device's last-level cache (256 MiB on MI355X): below that, inputs written
   // just before and outputs read just after are served from it.
@@ -1086,10 +1091,13 @@ void sync_adjoint_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, 
     const int grid = static_cast<int>(std::max<size_t>(
       1, std::min<size_t>(ceil_div(n, kGather), static_cast<size_t>(num_cus()) * env_int("RAGNAR_GATHER_SBPC", 1))));
     const size_t smem = kStageBytes + tables;
-    if (nt) {
-      binned_adj_gather<true, true, true><<<grid, kGather, smem, st>>>(D, n, Pg, U, G, kstage);
+    const bool e_terms = D.e[0] || D.e[1] || D.e[2] || G.ge[0] || G.ge[1] || G.ge[2];
+    if (e_terms) {
+      if (nt) binned_adj_gather<true, true, true><<<grid, kGather, smem, st>>>(D, n, Pg, U, G, kstage);
+      else binned_adj_gather<true, false, true><<<grid, kGather, smem, st>>>(D, n, Pg, U, G, kstage);
     } else {
-      binned_adj_gather<true, false, true><<<grid, kGather, smem, st>>>(D, n, Pg, U, G, kstage);
+      if (nt) binned_adj_gather<false, true, true><<<grid, kGather, smem, st>>>(D, n, Pg, U, G, kstage);
+      else binned_adj_gather<false, false, true><<<grid, kGather, smem, st>>>(D, n, Pg, U, G, kstage);
     }
   } else {
     const size_t bpc = static_cast<size_t>(env_int("RAGNAR_GATHER_BPC", 8));  // blocks per CU

@@ -388,6 +388,9 @@ struct ParticleGrads {
 };
 
 
+// E_TERMS = false: E is zero and dL/dE is not wanted (ge is left zero), so
+// the terms in E drop out.
+template <bool E_TERMS = true>
 __device__ __forceinline__ ParticleGrads emission_grads(const OmegaTerms& o, const RawParticle& r,
                                                         double A, double Cl, const EmitConst& C) {
   ParticleGrads g;
@@ -399,24 +402,36 @@ __device__ __forceinline__ ParticleGrads emission_grads(const OmegaTerms& o, con
   const double ggamma = -2.0 * w * chi * Cl * ig;
   const double coef = gchi * C.rB0 * rcp_nr(o.rootQ);
   const double ex = r.ex, ey = r.ey, ez = r.ez, bx = r.bx, by = r.by, bz = r.bz;
+  // V / (B0 rootQ) scaled by dL/dchi: the common factor of the terms below
+  // (fused multiply-adds throughout: the gradients need no correct rounding)
+  const double cx = coef * o.vx, cy = coef * o.vy, cz = coef * o.vz;
   // dchi/dB = (V x beta) / (B0 rootQ)
-  g.gb[0] = coef * (o.vy * o.btz - o.vz * o.bty);
-  g.gb[1] = coef * (o.vz * o.btx - o.vx * o.btz);
-  g.gb[2] = coef * (o.vx * o.bty - o.vy * o.btx);
-  // dchi/dE = (V - (beta.E) beta) / (B0 rootQ)
-  g.ge[0] = coef * (o.vx - o.bde * o.btx);
-  g.ge[1] = coef * (o.vy - o.bde * o.bty);
-  g.ge[2] = coef * (o.vz - o.bde * o.btz);
-  // dchi/dbeta = (B x V - (beta.E) E) / (B0 rootQ)
-  const double gbtx = coef * ((by * o.vz - bz * o.vy) - o.bde * ex);
-  const double gbty = coef * ((bz * o.vx - bx * o.vz) - o.bde * ey);
-  const double gbtz = coef * ((bx * o.vy - by * o.vx) - o.bde * ez);
+  g.gb[0] = fma(cy, o.btz, -cz * o.bty);
+  g.gb[1] = fma(cz, o.btx, -cx * o.btz);
+  g.gb[2] = fma(cx, o.bty, -cy * o.btx);
+  double gbtx, gbty, gbtz;
+  if constexpr (E_TERMS) {
+    const double cb = coef * o.bde;
+    // dchi/dE = (V - (beta.E) beta) / (B0 rootQ)
+    g.ge[0] = fma(-cb, o.btx, cx);
+    g.ge[1] = fma(-cb, o.bty, cy);
+    g.ge[2] = fma(-cb, o.btz, cz);
+    // dchi/dbeta = (B x V - (beta.E) E) / (B0 rootQ)
+    gbtx = fma(by, cz, fma(-bz, cy, -cb * ex));
+    gbty = fma(bz, cx, fma(-bx, cz, -cb * ey));
+    gbtz = fma(bx, cy, fma(-by, cx, -cb * ez));
+  } else {
+    g.ge[0] = g.ge[1] = g.ge[2] = 0.0;
+    gbtx = fma(by, cz, -bz * cy);
+    gbty = fma(bz, cx, -bx * cz);
+    gbtz = fma(bx, cy, -by * cx);
+  }
   const double ux = r.ux, uy = r.uy, uz = r.uz;
-  const double udg = ux * gbtx + uy * gbty + uz * gbtz;
-  const double t = ggamma - udg * ig * ig;
-  g.gu[0] = ig * (gbtx + ux * t);
-  g.gu[1] = ig * (gbty + uy * t);
-  g.gu[2] = ig * (gbtz + uz * t);
+  const double udg = fma(ux, gbtx, fma(uy, gbty, uz * gbtz));
+  const double t = fma(-udg * ig, ig, ggamma);
+  g.gu[0] = ig * fma(ux, t, gbtx);
+  g.gu[1] = ig * fma(uy, t, gbty);
+  g.gu[2] = ig * fma(uz, t, gbtz);
   return g;
 }
 
