@@ -269,6 +269,34 @@ def test_adjoint_methods_agree_large(orc, table, weighted):
         assert r.max() < 1e-2
 
 
+def test_adjoint_binned_without_e_field(orc, table):
+    """A store without E: the binned gather leaves out the E terms when dL/dE
+    is not asked for.  Its gradients equal those of the general form (dL/dE
+    asked for) bit for bit, and dL/dE at E = 0 agrees with the pair-by-pair
+    kernel's (the same bar as dL/du above)."""
+    n, nb = 1 << 20, 256
+    u, b = workloads.particles(n, seed=14)
+    bins = orc.logspace(1e-2, 1e7, nb)
+    plan = T.SyncPlan(table, bins)
+    U, B = dev(u), dev(b)
+    W = dev(np.random.default_rng(15).uniform(0.5, 2.0, n).astype(np.float32))
+    g = torch.tensor(_loss_weights(nb, seed=16), dtype=torch.float32, device=DEV)
+    lean = T.sync_adjoint(plan, U, B, W, None, g, need=("w", "u", "b"), method="binned")
+    full = T.sync_adjoint(plan, U, B, W, None, g, need=("w", "u", "b", "e"), method="binned")
+    assert lean[3] is None
+    for x, y in zip(lean[:3], full[:3]):
+        assert torch.equal(x, y)
+    d = T.sync_adjoint(plan, U, B, W, None, g, need=("e",), method="direct")[3]
+    x = d.double().cpu().numpy().ravel()
+    y = full[3].double().cpu().numpy().ravel()
+    r = np.abs(x - y) / np.abs(x).max()
+    print(f"adjoint dE binned vs direct without E (abs / max): max {r.max():.3e} "
+          f"p99.9 {np.percentile(r, 99.9):.3e}")
+    assert np.percentile(r, 99.9) < 1e-6
+    assert (r > 1e-5).mean() < 1e-4
+    assert r.max() < 1e-2
+
+
 @pytest.mark.parametrize("precision,tol", [(1, 1e-5), (2, 3e-3), (3, 1e-5)])
 @pytest.mark.parametrize("ng,offset", [(500, 0), (512, 0), (512, 1)])
 def test_contraction_split_fp16_rows(orc, table, precision, tol, ng, offset):
