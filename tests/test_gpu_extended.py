@@ -269,12 +269,14 @@ def test_adjoint_methods_agree_large(orc, table, weighted):
         assert r.max() < 1e-2
 
 
-def test_adjoint_binned_without_e_field(orc, table):
+@pytest.mark.parametrize("n", [1000, 3 * 1024 + 7, 1 << 20])
+def test_adjoint_binned_without_e_field(orc, table, n):
     """A store without E: the binned gather leaves out the E terms when dL/dE
     is not asked for.  Its gradients equal those of the general form (dL/dE
     asked for) bit for bit, and dL/dE at E = 0 agrees with the pair-by-pair
-    kernel's (the same bar as dL/du above)."""
-    n, nb = 1 << 20, 256
+    kernel's (the same bar as dL/du above).  Ragged sizes: one partial
+    workgroup, and a few workgroups with a remainder."""
+    nb = 256
     u, b = workloads.particles(n, seed=14)
     bins = orc.logspace(1e-2, 1e7, nb)
     plan = T.SyncPlan(table, bins)
