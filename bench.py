@@ -223,8 +223,8 @@ def summarize_compute(trace_dir, pmc_dir):
     for k, d in sorted(dur.items(), key=lambda kv: -statistics.mean(kv[1])):
         full = _demangle(k)
         short = _short_name(full)
-        if short not in ("forward_kernel", "combine_kernel", "mfma_contract", "mfma_contract2", "mfma_contract_bf16",
-                         "pack_kernel_matrix", "pack_kernel_matrix_bf16", "plaw_rows", "ic_kernel", "ic_combine"):
+        if short not in ("forward_kernel", "combine_kernel", "mfma_contract2", "mfma_contract_bf16",
+                         "pack_kernel_matrix", "pack_kernel_matrix_bf16", "ic_kernel", "ic_combine"):
             continue
         c = vals.get(k, {})
         t = statistics.mean(d[1:] if len(d) > 1 else d)
