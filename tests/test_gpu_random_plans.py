@@ -104,3 +104,35 @@ def test_random_plan_both_methods(orc, seed):
             else:
                 assert worst_far <= 1e-5, (what, m, k)
                 assert err.max() <= 2e-2, (what, m, k)
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_random_hist_plans(orc, seed):
+    """Energy histograms on seeded random grids: 1 to 3000 log or linear bins
+    with the low edge from 1e-3 to 10 and 0.5 to 6 decades of range, |u| or
+    sqrt(1 + u^2), momenta spilling past both ends.  Counts bit-exact and sums
+    to 1e-6 against the oracle."""
+    rng = np.random.default_rng(2000 + seed)
+    nb = int(rng.integers(1, 3001))
+    lo = 10 ** rng.uniform(-3, 1)
+    hi = lo * 10 ** rng.uniform(0.5, 6)
+    log_bins = bool(rng.random() < 0.6)
+    fourvel = bool(rng.random() < 0.6)
+    if nb == 1:
+        bins = np.array([lo], np.float32)
+    else:
+        bins = orc.logspace(lo, hi, nb) if log_bins else orc.linspace(lo, hi, nb)
+    n = int(10 ** rng.uniform(4, 5.5))
+    u, _ = workloads.particles(n, p=rng.uniform(-3, -1.2), umin=lo / 10, umax=hi * 10,
+                               seed=int(rng.integers(1 << 30)))
+    plan = T.HistPlan(bins, log_bins)
+    h, c = T.energy_hist(plan, _dev(u), fourvel)
+    h, c = h.cpu().numpy(), c.cpu().numpy()
+    _, hd, co = orc.energy_hist(u, bins, log_bins, fourvel)
+    what = f"seed {seed}: {nb} {'log' if log_bins else 'linear'} bins [{lo:.3g}, {hi:.3g}], fourvel {fourvel}, {n} particles"
+    print(what)
+    assert np.array_equal(c.astype(np.uint64), co), what
+    assert int(c.sum()) == n, what
+    fin = np.isfinite(hd)
+    assert np.array_equal(np.isfinite(h), fin), what
+    assert np.allclose(h[fin], hd[fin], rtol=1e-6, atol=0), what
