@@ -53,6 +53,22 @@ def _dev(a):
     return None if a is None else torch.from_numpy(a).cuda()
 
 
+def _check_spectrum(S, ex, f64, what):
+    """Parity and accuracy to 1e-5 (tests/_spectra.py); a bin where the
+    reference's own FP32 result is more than 1e-5 from the FP64 model is held
+    to 4 times that discrepancy (at least 1e-5) from the nearer of the two."""
+    parity, acc, live, own, _ = spectrum_errors(S, ex, f64)
+    print(f"{what}: parity {parity:.2e} accuracy(FP64) {acc:.2e} over {int(live.sum())} bins")
+    assert parity <= TOL and acc <= TOL, (what, parity, acc)
+    for i in own:
+        d_ref = abs(ex[i] - f64[i]) / abs(ex[i])
+        d_gpu = min(abs(S[i] - ex[i]) / abs(ex[i]), abs(S[i] - f64[i]) / abs(f64[i]))
+        print(f"  ill-conditioned bin {i} ({ex[i] / np.abs(ex).max():.1e} of the peak): reference FP32 vs "
+              f"FP64 {d_ref:.1e}, GPU to the nearer {d_gpu:.1e}")
+        assert d_gpu <= max(TOL, 4.0 * d_ref), (what, i)
+    assert np.all(np.abs(S[~live]) <= 1e-10 * np.abs(ex).max() + 1e-37), what
+
+
 @pytest.mark.parametrize("seed", range(12))
 def test_random_plan_both_methods(orc, seed):
     bins, (B0, g_syn, e_at), u, b, e, w, g, need = _case(1000 + seed)
@@ -67,16 +83,7 @@ def test_random_plan_both_methods(orc, seed):
     S = {}
     for m in ("direct", "binned"):
         S[m] = T.sync_spectrum(plan, U, Bt, W, E, method=m).double().cpu().numpy()
-        parity, acc, live, own, _ = spectrum_errors(S[m], ex, f64)
-        print(f"{what}, {m}: parity {parity:.2e} accuracy(FP64) {acc:.2e} over {int(live.sum())} bins")
-        assert parity <= TOL and acc <= TOL, (what, m, parity, acc)
-        for i in own:
-            d_ref = abs(ex[i] - f64[i]) / abs(ex[i])
-            d_gpu = min(abs(S[m][i] - ex[i]) / abs(ex[i]), abs(S[m][i] - f64[i]) / abs(f64[i]))
-            print(f"  ill-conditioned bin {i} ({ex[i] / ex.max():.1e} of the peak): reference FP32 vs FP64 "
-                  f"{d_ref:.1e}, GPU to the nearer {d_gpu:.1e}")
-            assert d_gpu <= max(TOL, 4.0 * d_ref), (what, m, i)
-        assert np.all(np.abs(S[m][~live]) <= 1e-10 * np.abs(ex).max() + 1e-37), (what, m)
+        _check_spectrum(S[m], ex, f64, f"{what}, {m}")
     live = S["direct"] > 1e-6 * S["direct"].max()
     if live.any():
         assert np.max(np.abs(S["binned"] - S["direct"])[live] / S["direct"][live]) < 5e-6, what
@@ -136,3 +143,28 @@ def test_random_hist_plans(orc, seed):
     fin = np.isfinite(hd)
     assert np.array_equal(np.isfinite(h), fin), what
     assert np.allclose(h[fin], hd[fin], rtol=1e-6, atol=0), what
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_fromdist(orc, seed):
+    """SynchrotronSpectrumFromDist on seeded random grids: 2 to 4000 log or
+    linear distribution bins over 0.5 to 5 decades from gamma beta 1 to 10,
+    a noisy power law, random g_syn, e_at and photon grid; parity and
+    accuracy against the oracle as for the particle form."""
+    rng = np.random.default_rng(3000 + seed)
+    nb = int(rng.integers(8, 513))
+    lo, span = rng.uniform(-4, 2), rng.uniform(1, 9)
+    bins = np.logspace(lo, lo + span, nb).astype(np.float32)
+    g_syn, e_at = float(rng.uniform(0.5, 3)), float(10 ** rng.uniform(-1, 1))
+    ng = int(rng.integers(2, 4001))
+    glo = 10 ** rng.uniform(0, 1)
+    ghi = glo * 10 ** rng.uniform(0.5, 5)
+    dist_log = bool(rng.random() < 0.6)
+    g = orc.logspace(glo, ghi, ng) if dist_log else orc.linspace(glo, ghi, ng)
+    f = (g.astype(np.float64) ** -rng.uniform(1.2, 3.5) * rng.uniform(0.7, 1.3, ng)).astype(np.float32)
+    plan = T.SyncPlan(T.default_table(), bins, 1.0, g_syn, e_at)
+    S = T.sync_fromdist(plan, _dev(g), _dev(f), dist_log).double().cpu().numpy()
+    _, ex = orc.sync_fromdist(g, f, dist_log, bins, g_syn=g_syn, e_at=e_at, want_fp64=True)
+    _, f64 = orc.sync_fromdist(g, f, dist_log, bins, g_syn=g_syn, e_at=e_at, mode=orc.FP64, want_fp64=True)
+    _check_spectrum(S, ex, f64, f"seed {seed}: {ng} {'log' if dist_log else 'linear'} bins "
+                                f"[{glo:.3g}, {ghi:.3g}], {nb} photon bins [{bins[0]:.1e}, {bins[-1]:.1e}]")
