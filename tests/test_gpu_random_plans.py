@@ -198,3 +198,47 @@ def test_random_ic(orc, seed):
     print(f"{what}: max rel err vs reference terms {err:.2e} over {int(live.sum())} bins")
     assert err < 1e-5, what
     assert np.array_equal(S[ex == 0], ex[ex == 0]), what
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_random_cells(orc, seed):
+    """Batched cells on seeded random plans and layouts: 1 to 200 cells of
+    log-normal sizes with empty and one-particle cells, with and without E
+    and weights.  Every cell against the single-call form of its slice (the
+    form checked against the oracle above) to 2e-6 over bins above 1e-12 of
+    its peak, empty cells exactly zero, a repeated call bit for bit."""
+    rng = np.random.default_rng(5000 + seed)
+    nb = int(rng.integers(8, 513))
+    lo, span = rng.uniform(-4, 2), rng.uniform(1, 9)
+    bins = np.logspace(lo, lo + span, nb).astype(np.float32)
+    B0, g_syn, e_at = float(10 ** rng.uniform(-1, 1)), float(rng.uniform(0.5, 3)), float(10 ** rng.uniform(-1, 1))
+    ncells = int(rng.integers(1, 201))
+    sizes = np.floor(rng.lognormal(np.log(500), 1.2, ncells)).astype(np.int64)
+    sizes[rng.random(ncells) < 0.05] = 0
+    sizes[rng.random(ncells) < 0.05] = 1
+    offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    n = int(offsets[-1])
+    if n == 0:
+        offsets[-1] = n = 1
+    u, b = workloads.particles(n, p=rng.uniform(-3.5, -1.5), umax=10 ** rng.uniform(1, 4),
+                               seed=int(rng.integers(1 << 30)))
+    e = rng.normal(0, 0.2, (3, n)).astype(np.float32) if rng.random() < 0.4 else None
+    w = rng.uniform(0.1, 3, n).astype(np.float32) if rng.random() < 0.5 else None
+    plan = T.SyncPlan(T.default_table(), bins, B0, g_syn, e_at)
+    U, Bt, E, W = _dev(u), _dev(b), _dev(e), _dev(w)
+    what = (f"seed {seed}: {ncells} cells, {n} particles, {nb} bins [{bins[0]:.1e}, {bins[-1]:.1e}], "
+            f"E {e is not None}, w {w is not None}")
+    print(what)
+    S = T.sync_cells(plan, U, Bt, offsets, W, E).cpu().numpy()
+    assert np.array_equal(S, T.sync_cells(plan, U, Bt, offsets, W, E).cpu().numpy()), what
+    sl = lambda t, a, z: None if t is None else t[..., a:z].contiguous()
+    for c in range(ncells):
+        a, z = int(offsets[c]), int(offsets[c + 1])
+        if z == a:
+            assert not np.any(S[c]), (what, c)
+            continue
+        single = T.sync_spectrum(plan, sl(U, a, z), sl(Bt, a, z), sl(W, a, z), sl(E, a, z)).cpu().numpy()
+        live = single > 1e-12 * single.max() if single.max() > 0 else np.zeros(nb, bool)
+        if live.any():
+            assert np.max(np.abs(S[c][live] / single[live] - 1)) < 2e-6, (what, c)
+        assert np.all(np.abs(S[c][~live]) <= 1e-10 * max(single.max(), 0.0) + 1e-37), (what, c)
