@@ -33,6 +33,7 @@
 // switches from segment m to m - 1, and every particle reads one entry.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <vector>
@@ -1085,8 +1086,10 @@ void sync_adjoint_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, 
   const bool nt = sizeof(float) * streams * n > (size_t{256} << 20);
   // staged rows: as many as kStageBytes holds (and the histogram of NK buckets fits)
   const int kstage = static_cast<int>(std::min<size_t>(plan.NK, kStageBytes / (sizeof(double2) * (plan.M + 1))));
+  // RAGNAR_GATHER_STAGE=0 reads every U entry from the table in memory
+  const char* stage_env = std::getenv("RAGNAR_GATHER_STAGE");
   const bool stage = Pg.NK > 0 && kstage >= 1 && sizeof(unsigned) * plan.NK <= kStageBytes &&
-                     env_int("RAGNAR_GATHER_STAGE", 1) == 1;
+                     !(stage_env && std::strcmp(stage_env, "0") == 0);
   if (stage) {
     const int grid = static_cast<int>(std::max<size_t>(
       1, std::min<size_t>(ceil_div(n, kGather), static_cast<size_t>(num_cus()) * env_int("RAGNAR_GATHER_SBPC", 1))));

@@ -242,3 +242,43 @@ def test_random_cells(orc, seed):
         if live.any():
             assert np.max(np.abs(S[c][live] / single[live] - 1)) < 2e-6, (what, c)
         assert np.all(np.abs(S[c][~live]) <= 1e-10 * max(single.max(), 0.0) + 1e-37), (what, c)
+
+
+def test_switch_paths_match_default(orc, monkeypatch):
+    """The launch shapes that only environment switches select give the
+    default path's results: the adjoint gather without staged rows
+    (RAGNAR_GATHER_STAGE=0) and with two staged workgroups per CU
+    (RAGNAR_GATHER_SBPC=2) bit for bit (the same U entries are read); the
+    forward with 8-wide sub-histogram batches (RAGNAR_SUBHIST_BATCH=8) and
+    other work-item sizes (RAGNAR_SUBHIST_ITEM) to 1e-6 (the FP64 moments
+    are summed in another order)."""
+    n, nb = 1 << 20, 256
+    u, b = workloads.particles(n, seed=31)
+    w = np.random.default_rng(32).uniform(0.5, 2.0, n).astype(np.float32)
+    plan = T.SyncPlan(T.default_table(), orc.logspace(1e-2, 1e7, nb))
+    U, Bt, W = _dev(u), _dev(b), _dev(w)
+    g = torch.from_numpy(np.random.default_rng(33).normal(0, 1, nb).astype(np.float32)).cuda()
+
+    def adj():
+        return [x.clone() for x in T.sync_adjoint(plan, U, Bt, W, None, g, need=("w", "u", "b"),
+                                                  method="binned")[:3]]
+
+    def fwd():
+        return T.sync_spectrum(plan, U, Bt, W, method="binned").double().cpu().numpy()
+
+    ref_a, ref_f = adj(), fwd()
+    for name, val in (("RAGNAR_GATHER_STAGE", "0"), ("RAGNAR_GATHER_SBPC", "2")):
+        monkeypatch.setenv(name, val)
+        for x, y in zip(adj(), ref_a):
+            assert torch.equal(x, y), (name, val)
+        monkeypatch.delenv(name)
+    live = ref_f > 1e-12 * ref_f.max()
+    for name, val in (("RAGNAR_SUBHIST_BATCH", "8"), ("RAGNAR_SUBHIST_ITEM", "8192"),
+                      ("RAGNAR_SUBHIST_ITEM", "100000")):
+        monkeypatch.setenv(name, val)
+        S = fwd()
+        err = float(np.max(np.abs(S[live] / ref_f[live] - 1)))
+        print(f"{name}={val}: forward {err:.2e} from the default")
+        assert err <= 1e-6, (name, val)
+        assert np.array_equal(S[~live], ref_f[~live]), (name, val)
+        monkeypatch.delenv(name)
