@@ -1076,8 +1076,8 @@ void sync_adjoint_binned(const SyncPlan& plan, const ParticlesDev& D, size_t n, 
   // The unstaged form is the general one also for stores without E: the
   // shorter one measured ~4% slower there (more resident waves, same memory
   // traffic).  The staged form drops the E terms when the store has no E and
-  // dL/dE is not wanted (35 of 253 vector instructions per particle,
-  // 195 -> 188 us at 2^24 particles).  Non-temporal
+  // dL/dE is not wanted (35 of the 253 vector instructions of the compiled
+  // loop body, 195 -> 188 us at 2^24 particles).  Non-temporal
   // streams pay off once a call's particles and gradients outgrow the
   // device's last-level cache (256 MiB on MI355X): below that, inputs written
   // just before and outputs read just after are served from it.
